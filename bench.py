@@ -1,0 +1,210 @@
+"""Benchmark: SVD solves/s and p50/p99 latency of the one-sided Jacobi SVD on one MI355X.
+
+Default workload (one step = one full SVD solve, input resident in HBM, outputs U, sigma, V in HBM):
+  tall_geometric 4096 x 1024, A = Q1 diag(sigma) Q2^T with sigma geometric from 1 to 1e-8,
+  plain FP64 cyclic one-sided Jacobi, eps 1e-15, at most 30 sweeps.
+
+Prints ONE JSON line with the metric, the roofline of the dominant kernel (the fused round kernel,
+timed with HIP events on the stream it runs on) and a CPU baseline (the oracle's C restatement of the
+reference algorithm, OpenMP over the pairs of a round, on a bounded sample of sweeps).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tall|plain] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jacobi-svd-cuda_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from jsvd import workloads
+    if name == "tall":
+        a, sigma = workloads.tall_geometric()
+        return a, sigma, 1e-15, 30, "tall_geometric_4096x1024_fp64"
+    if name == "plain":
+        a = workloads.plain_gaussian()
+        return a, None, 1e-15, 30, "plain_gaussian_512x256_fp64"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
+    """Times the fused round kernel with HIP events around each launch on the workload's own data
+    (first sweep: every pair rotates).  Returns (avg_us, algorithmic bytes per launch)."""
+    m, n = a_host.shape
+    lda, ldv = m + (m % 2), n + (n % 2)
+    stream = torch.cuda.current_stream()
+    acol = torch.zeros((n, lda), dtype=torch.float64, device="cuda")
+    acol[:, :m] = torch.from_numpy(a_host.T.copy()).to("cuda")
+    vcol = torch.zeros((n, ldv), dtype=torch.float64, device="cuda")
+    vcol[:, :n] = torch.eye(n, dtype=torch.float64, device="cuda")
+    rot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    N = n + (n % 2)
+    rounds = N - 1
+    launches = min(launches, rounds)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    torch.cuda.synchronize()
+    for r in range(launches):
+        starts[r].record(stream)
+        jsvd.round_(acol, lda, m, vcol, ldv, n, r, eps, rot_count=rot, stream=stream)
+        ends[r].record(stream)
+    torch.cuda.synchronize()
+    total_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends))
+    rotated = int(rot.item())
+    pairs = launches * (n // 2)
+    # fused kernel: every pair reads its two A columns (Gram); a rotated pair also writes them and
+    # reads + writes its two V columns.
+    algo_bytes = 8 * (pairs * 2 * m + rotated * (2 * m + 4 * n))
+    return 1000.0 * total_ms / launches, algo_bytes / launches, rotated, pairs
+
+
+def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+    t0 = time.perf_counter()
+    res = oracle.svd(a_host, eps=eps, max_sweeps=sample_sweeps, threads=threads)
+    dt = time.perf_counter() - t0
+    per_sweep = dt / res.sweeps
+    solves_per_s = 1.0 / (per_sweep * gpu_sweeps)
+    m, n = a_host.shape
+    return {
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle C restatement, OpenMP {threads} threads, {res.sweeps} sweeps of the {m}x{n} workload "
+                  f"in {dt:.2f}s ({per_sweep:.3f}s/sweep), scaled to the {gpu_sweeps} sweeps a full solve takes",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tall", choices=["tall", "plain"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this solver runs on one GPU; multi-GPU runs are not supported",
+                          "n_gpus": args.gpus}))
+        raise SystemExit(2)
+
+    import torch
+    import jsvd
+
+    log(f"[bench] building workload {args.workload}")
+    a_host, sigma_true, eps, max_sweeps, wname = make_workload(args.workload)
+    m, n = a_host.shape
+    cfg = jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=max_sweeps)
+    torch.cuda.init()
+    d_a = torch.from_numpy(a_host).to("cuda")
+    d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    plan = jsvd.Plan(m, n, cfg)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        plan.solve_device(d_a, d_u, d_s, d_v)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    lat = []
+    sweeps = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        sweeps = plan.solve_device(d_a, d_u, d_s, d_v)   # returns after the solve completed
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    lat_ms = np.array(lat) * 1e3
+
+    sig = d_s.cpu().numpy()
+    u = d_u.cpu().numpy()
+    v = d_v.cpu().numpy()
+    acc = {
+        "sweeps": int(sweeps),
+        "rho_A": float(np.linalg.norm(a_host - (u * sig) @ v.T) / max(1.0, np.linalg.norm(a_host))),
+        "rho_U": float(np.linalg.norm(u.T @ u - np.eye(n))),
+        "rho_V": float(np.linalg.norm(v.T @ v - np.eye(n))),
+    }
+    if sigma_true is not None:
+        s_sorted = np.sort(sig)[::-1]
+        acc["sigma_err_normwise"] = float(np.max(np.abs(s_sorted - sigma_true)) / sigma_true[0])
+        acc["sigma_relerr_max"] = float(np.max(np.abs(s_sorted - sigma_true) / sigma_true))
+    log(f"[bench] accuracy {acc}")
+
+    log("[bench] round-kernel roofline")
+    avg_us, bytes_per_launch, rotated, pairs = round_kernel_roofline(torch, jsvd, a_host, eps)
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    roofline = {
+        "kernel": "round_kernel (fused Gram + rotation + Givens apply, one launch per round)",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "avg_launch_us": avg_us,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "rotated_pairs": rotated,
+        "pairs": pairs,
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(a_host, eps, sweeps, args.cpu_sample_sweeps)
+
+    out = {
+        "metric": "SVD solves/s + p99 latency (plain FP64 one-sided Jacobi)",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wname, "m": m, "n": n, "epsilon": eps, "max_sweeps": max_sweeps,
+                   "path": "plain_f64"},
+        "accuracy": acc,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * jsvd.h -- C ABI of the MI355X (gfx950) one-sided Jacobi SVD library, libjsvd.so.
+ *
+ * This is the drop-in boundary for the reference's plain FP64 solver (kleedaisuki/jacobi-svd-cuda).
+ * The C++ entry points of include/kernels.hpp (namespace jacobi::svd) are thin wrappers over these
+ * functions; a ctypes / FFI binding binds these symbols directly.  All pointers are plain host or
+ * device pointers, all sizes are int64_t, and every function returns a status code:
+ *
+ *   JSVD_OK (0)                    success
+ *   JSVD_ERR_INVALID_ARGUMENT (1)  precondition violated (maps to std::invalid_argument)
+ *   JSVD_ERR_DEVICE (2)            HIP runtime failure      (maps to jacobi::svd::CudaError)
+ *   JSVD_ERR_INTERNAL (3)          anything else            (maps to std::runtime_error)
+ *
+ * jsvd_last_error() returns the thread-local message of the last failing call on this thread.
+ *
+ * Matrices crossing the boundary are row-major, exactly as in the reference: A is m x n, U is
+ * m x n, V is n x n, sigma has n entries in Jacobi column order (not sorted), m >= n.
+ */
+#ifndef JSVD_H
+#define JSVD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JSVD_OK 0
+#define JSVD_ERR_INVALID_ARGUMENT 1
+#define JSVD_ERR_DEVICE 2
+#define JSVD_ERR_INTERNAL 3
+
+/* layout_transpose_mode values, mirroring jacobi::svd::LayoutTransposeMode (include/kernels.hpp:13-29). */
+#define JSVD_LAYOUT_AUTO 0
+#define JSVD_LAYOUT_FORCE_ENABLE 1
+#define JSVD_LAYOUT_FORCE_DISABLE 2
+
+/*
+ * Solver configuration.  Field-for-field the reference's JacobiSvdConfig (include/kernels.hpp:78-124),
+ * same defaults (jsvd_config_default).  threads_per_block is validated and normalised as the reference
+ * does (clamped to [32,1024], rounded up to a multiple of 32; src/kernels.cu:685-690) and reported, but the
+ * launch shape itself is chosen per matrix shape in wave64 units.
+ */
+typedef struct jsvd_config
+{
+    double epsilon;                                 /* kernels.hpp:83, default 1e-9 */
+    int32_t max_sweeps;                             /* kernels.hpp:88, default 128 */
+    int32_t threads_per_block;                      /* kernels.hpp:93, default 256 */
+    int32_t layout_transpose_mode;                  /* kernels.hpp:98, JSVD_LAYOUT_* */
+    int32_t layout_transpose_min_columns;           /* kernels.hpp:103, default 16 */
+    uint64_t layout_transpose_min_elements;         /* kernels.hpp:108, default 4096 */
+    int32_t layout_transpose_auto_tune;             /* kernels.hpp:113, default 0 */
+    int32_t layout_transpose_benchmark_repetitions; /* kernels.hpp:118, default 2 */
+    int32_t layout_transpose_benchmark_sweeps;      /* kernels.hpp:123, default 8 */
+    int32_t reserved;
+} jsvd_config;
+
+/* Fills *cfg with the reference defaults (include/kernels.hpp:83-123). */
+void jsvd_config_default(jsvd_config *cfg);
+
+/* Thread-local message of the last failing call on the calling thread ("" if none). */
+const char *jsvd_last_error(void);
+
+/* Library version string. */
+const char *jsvd_version(void);
+
+/*
+ * Plain FP64 one-sided Jacobi SVD of a host row-major m x n matrix.
+ * Replaces jacobi::svd::one_sided_jacobi_svd (include/kernels.hpp:295-298, src/kernels.cu:1066-1076):
+ * same validation (src/kernels.cu:807-833), pair ordering (src/kernels.cu:637-678), rotation and skip
+ * test (src/kernels.cu:485-521), convergence rule and sweep count (src/kernels.cu:889-981) and sigma/U
+ * extraction (src/kernels.cu:586-629).  u (m*n), sigma (n) and v (n*n) are host buffers owned by the
+ * caller; sweeps receives the executed sweep count (== max_sweeps when not converged, not an error).
+ */
+int jsvd_plain_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg,
+                   double *u, double *sigma, double *v, int32_t *sweeps);
+
+/*
+ * Reusable solver plan for one shape: owns the device workspace and a HIP stream, so repeated solves of
+ * the same shape allocate nothing.  No reference counterpart (the reference allocates per call,
+ * src/kernels.cu:854-885); it backs jsvd_plain_f64 and the device-resident entry point below.
+ * stream may be NULL (the plan then creates its own non-blocking stream) or a hipStream_t.
+ */
+typedef struct jsvd_plan jsvd_plan;
+int jsvd_plan_create(int64_t m, int64_t n, const jsvd_config *cfg, void *stream, jsvd_plan **plan);
+int jsvd_plan_destroy(jsvd_plan *plan);
+
+/*
+ * Device-resident solve: d_a (row-major m x n), d_u (m x n), d_sigma (n), d_v (n x n) are device
+ * pointers.  d_a is not modified.  Work is enqueued on the plan's stream; the call returns after the
+ * solve completes (the host must learn the sweep count).  Same semantics as jsvd_plain_f64.
+ */
+int jsvd_plan_solve_device(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                           int32_t *sweeps);
+
+/* Host-pointer solve through a plan (H2D copy, device solve, D2H copies). */
+int jsvd_plan_solve_host(jsvd_plan *plan, const double *a, double *u, double *sigma, double *v, int32_t *sweeps);
+
+/*
+ * Per-sweep diagnostics of the last solve on this plan: rotated-pair count and the largest
+ * |gamma|/sqrt(alpha*beta) over all pairs of each sweep (the off-orthogonality measure of the
+ * convergence test).  Arrays of length at least `capacity`; returns the number of sweeps written.
+ */
+int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *offmax, int32_t capacity);
+
+/*
+ * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
+ * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the
+ * pair of the dummy column when n is odd, and returns the pair count (negative status on error).
+ * Equals build_round_robin_schedule (src/kernels.cu:637-678) round by round, orientation included.
+ */
+int jsvd_round_robin_pairs(int64_t n, int32_t round, int32_t *pairs_out);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Kernel-level entry points (device pointers, enqueued on `stream`, asynchronous).  They expose the
+ * four kernels of the plain path for parity tests and roofline measurement.
+ * ------------------------------------------------------------------------------------------------- */
+
+/*
+ * Out-of-place transpose of a row-major rows x cols matrix src (leading dimension lds >= cols) into
+ * dst as row-major cols x rows (leading dimension ldd >= rows).  LDS-tiled, coalesced both ways.
+ * Replaces transpose_row_major_kernel and its launchers (src/kernels.cu:246-321).
+ */
+int jsvd_transpose_f64(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols,
+                       void *stream);
+
+/*
+ * Column-pair Gram statistics: for each k < count, with (p,q) = pairs[2k], pairs[2k+1],
+ * alpha[k] = |a_p|^2, beta[k] = |a_q|^2, gamma[k] = a_p . a_q over the m rows of the column-major
+ * matrix a (leading dimension lda).  Replaces pair_stats_kernel (src/kernels.cu:412-472).
+ * pairs is a device int32 array.
+ */
+int jsvd_pair_gram_f64(const double *a, int64_t lda, int64_t m, const int32_t *pairs, int64_t count,
+                       double *alpha, double *beta, double *gamma, void *stream);
+
+/*
+ * One fused round of the plain path: for every pair (p,q) of round `round` of the round-robin schedule
+ * of n columns, compute (alpha, beta, gamma), the rotation (c, s) in FP64 with the reference's skip
+ * test, and apply it to columns p, q of the column-major a (m rows, lda) and v (n rows, ldv).
+ * Replaces pair_stats_kernel + compute_rotation_params_kernel + apply_rotation_kernel of one round
+ * (src/kernels.cu:908-965).  rot_count (device int32, may be NULL) is incremented once per rotated
+ * pair; offmax (device uint64 holding a non-negative double's bits, may be NULL) receives the max of
+ * |gamma|/sqrt(alpha*beta).  If skip_if_zero is non-NULL and *skip_if_zero == 0 on entry the kernel
+ * does nothing (device-side early exit used after convergence).
+ */
+int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t round,
+                   double epsilon, int32_t *rot_count, uint64_t *offmax, const int32_t *skip_if_zero,
+                   void *stream);
+
+/*
+ * Rotation parameters and skip test for `count` pairs: rotate[k] = 1 and (c[k], s[k]) from alpha, beta,
+ * gamma in FP64 when |gamma| > epsilon*sqrt(max(alpha,0)*max(beta,0)) and |gamma| > 1e-300, else
+ * rotate[k] = 0, c = 1, s = 0.  Replaces compute_rotation_params_kernel (src/kernels.cu:485-521); the
+ * per-pair flag array replaces its single atomicExch flag.  Evaluated without FMA contraction, so the
+ * result equals an IEEE double evaluation of the same formula bit for bit.
+ */
+int jsvd_rotation_params_f64(const double *alpha, const double *beta, const double *gamma, int64_t count,
+                             double epsilon, double *c, double *s, int32_t *rotate, void *stream);
+
+/*
+ * Two-stage off-orthogonality reduction: out[0] = max over pairs i<j of |a_i . a_j| / (|a_i| |a_j|) for
+ * the column-major a (m x n, lda) (pairs with a zero column are skipped).  Stage one reduces row
+ * tiles into per-block partial maxima, stage two reduces the partials.  No reference counterpart as a
+ * kernel (the reference only sets a flag, src/kernels.cu:514); it is the convergence measure of the
+ * skip test.
+ */
+int jsvd_offnorm_f64(const double *a, int64_t lda, int64_t m, int64_t n, double *out, void *stream);
+
+/*
+ * sigma/U extraction: sigma_j = sqrt(sum_i a_ij^2), U[:,j] = a_j / sigma_j if sigma_j > epsilon else 0,
+ * from the column-major a (m x n, lda) into row-major u (m x n).  Replaces build_u_sigma_kernel
+ * (src/kernels.cu:586-629) fused with the final column-to-row transpose (src/kernels.cu:983-988).
+ */
+int jsvd_u_sigma_f64(const double *a, int64_t lda, int64_t m, int64_t n, double epsilon, double *u,
+                     double *sigma, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JSVD_H */

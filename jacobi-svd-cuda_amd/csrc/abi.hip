@@ -1,0 +1,520 @@
+// abi.hip -- the C ABI of libjsvd (include/jsvd.h): configuration, validation, solver plans and the
+// device-resident sweep driver of the plain FP64 path.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "plain_f64.hpp"
+
+namespace jsvd
+{
+    namespace
+    {
+        thread_local std::string t_last_error;
+    }
+
+    void set_last_error(const std::string &message) { t_last_error = message; }
+    void clear_last_error() { t_last_error.clear(); }
+
+    void throw_hip(hipError_t status, const char *expression, const char *file, int line)
+    {
+        std::ostringstream stream;
+        stream << "HIP call failed: " << expression << " @ " << file << ':' << line
+               << ", code=" << static_cast<int>(status) << ", message=" << hipGetErrorString(status);
+        throw Error(JSVD_ERR_DEVICE, stream.str());
+    }
+
+    // src/kernels.cu:338-356 (layout knobs) and 823-832 (epsilon, max_sweeps).
+    void validate_config(const jsvd_config &cfg)
+    {
+        if (!(cfg.epsilon > 0.0))
+        {
+            throw_invalid("epsilon must be positive.");
+        }
+        if (cfg.max_sweeps <= 0)
+        {
+            throw_invalid("max_sweeps must be positive.");
+        }
+        if (cfg.layout_transpose_min_columns <= 0)
+        {
+            throw_invalid("layout_transpose_min_columns must be positive.");
+        }
+        if (cfg.layout_transpose_min_elements == 0)
+        {
+            throw_invalid("layout_transpose_min_elements must be positive.");
+        }
+        if (cfg.layout_transpose_benchmark_repetitions <= 0)
+        {
+            throw_invalid("layout_transpose_benchmark_repetitions must be positive.");
+        }
+        if (cfg.layout_transpose_benchmark_sweeps <= 0)
+        {
+            throw_invalid("layout_transpose_benchmark_sweeps must be positive.");
+        }
+        if (cfg.layout_transpose_mode < JSVD_LAYOUT_AUTO || cfg.layout_transpose_mode > JSVD_LAYOUT_FORCE_DISABLE)
+        {
+            throw_invalid("layout_transpose_mode is not a valid LayoutTransposeMode.");
+        }
+    }
+
+    // src/kernels.cu:813-820.
+    void validate_shape(int64_t m, int64_t n)
+    {
+        if (m <= 0 || n <= 0)
+        {
+            throw_invalid("Input matrix shape must be non-zero.");
+        }
+        if (m < n)
+        {
+            throw_invalid("One-sided Jacobi SVD requires rows >= columns in this implementation.");
+        }
+        if (m > (int64_t{1} << 31) || n > (int64_t{1} << 30))
+        {
+            throw_invalid("Input matrix shape exceeds the supported range.");
+        }
+    }
+
+    int normalize_threads_per_block(int raw)
+    {
+        const int clamped = std::clamp(raw, 32, 1024);
+        return std::min(((clamped + 31) / 32) * 32, 1024);
+    }
+} // namespace jsvd
+
+// The opaque plan: one shape, its device workspace, a stream, and the pinned mailbox through which the
+// host learns each sweep's rotation count without a blocking copy.
+struct jsvd_plan
+{
+    int64_t m = 0;
+    int64_t n = 0;
+    int64_t lda = 0; // even, >= m
+    int64_t ldv = 0; // even, >= n
+    jsvd_config cfg{};
+    jsvd::RoundShape shape{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double *a_col = nullptr;            // lda x n column-major working copy of A
+    double *v_col = nullptr;            // ldv x n column-major V
+    int *rot_count = nullptr;           // [max_sweeps] rotated pairs per sweep
+    unsigned long long *offmax = nullptr; // [max_sweeps] max pair cosine per sweep (double bits)
+    int *h_rot = nullptr;               // pinned mirror of rot_count
+    hipEvent_t sweep_done[2] = {nullptr, nullptr};
+    // Host-API staging (allocated on first host solve).
+    double *d_a = nullptr;
+    double *d_u = nullptr;
+    double *d_sigma = nullptr;
+    double *d_v = nullptr;
+    // Diagnostics of the last solve.
+    int last_sweeps = 0;
+    std::vector<int> last_rot;
+    std::vector<double> last_off;
+
+    ~jsvd_plan()
+    {
+        (void)hipFree(a_col);
+        (void)hipFree(v_col);
+        (void)hipFree(rot_count);
+        (void)hipFree(offmax);
+        (void)hipHostFree(h_rot);
+        (void)hipFree(d_a);
+        (void)hipFree(d_u);
+        (void)hipFree(d_sigma);
+        (void)hipFree(d_v);
+        for (hipEvent_t e : sweep_done)
+        {
+            if (e)
+            {
+                (void)hipEventDestroy(e);
+            }
+        }
+        if (own_stream && stream)
+        {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void create(int64_t rows, int64_t cols, const jsvd_config &config, hipStream_t user_stream)
+    {
+        m = rows;
+        n = cols;
+        cfg = config;
+        lda = jsvd::round_up(m, 2);
+        ldv = jsvd::round_up(n, 2);
+        shape = jsvd::choose_round_shape(m, n);
+        if (user_stream)
+        {
+            stream = user_stream;
+        }
+        else
+        {
+            JSVD_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        JSVD_HIP_CHECK(hipMalloc(&a_col, sizeof(double) * lda * n));
+        JSVD_HIP_CHECK(hipMalloc(&v_col, sizeof(double) * ldv * n));
+        JSVD_HIP_CHECK(hipMalloc(&rot_count, sizeof(int) * cfg.max_sweeps));
+        JSVD_HIP_CHECK(hipMalloc(&offmax, sizeof(unsigned long long) * cfg.max_sweeps));
+        JSVD_HIP_CHECK(hipHostMalloc(&h_rot, sizeof(int) * cfg.max_sweeps, hipHostMallocDefault));
+        for (hipEvent_t &e : sweep_done)
+        {
+            JSVD_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        // Padding rows m..lda-1 stay zero for the plan's lifetime: transposes only write rows < m and the
+        // round kernel reads/writes the padding row of an odd m as zero.
+        JSVD_HIP_CHECK(hipMemsetAsync(a_col, 0, sizeof(double) * lda * n, stream));
+    }
+
+    // Device-resident solve: transpose in, sweep loop, sigma/U extraction, V transpose out.
+    void solve_device(const double *a_rm, double *u_rm, double *sigma, double *v_rm, int *sweeps_out)
+    {
+        const int max_sweeps = cfg.max_sweeps;
+        const double eps = cfg.epsilon;
+        jsvd::launch_transpose(a_rm, n, a_col, lda, m, n, stream);
+        jsvd::launch_identity(v_col, ldv, n, stream);
+        JSVD_HIP_CHECK(hipMemsetAsync(rot_count, 0, sizeof(int) * max_sweeps, stream));
+        JSVD_HIP_CHECK(hipMemsetAsync(offmax, 0, sizeof(unsigned long long) * max_sweeps, stream));
+        std::fill(h_rot, h_rot + max_sweeps, -1);
+
+        const int N = static_cast<int>(n + (n & 1));
+        const int rounds = (n >= 2) ? N - 1 : 0;
+        int enqueued = 0;
+        // Sweep s+1 is enqueued before the host looks at sweep s, so the device never idles on the host;
+        // every round of a sweep that follows a rotation-free sweep exits on the device flag.  The host
+        // only waits on the event of sweep s-1, which completed when sweep s began.
+        for (int s = 0; s < max_sweeps; ++s)
+        {
+            const int *skip = (s > 0) ? rot_count + (s - 1) : nullptr;
+            for (int r = 0; r < rounds; ++r)
+            {
+                jsvd::launch_round(shape, a_col, lda, m, v_col, ldv, n, r, eps, rot_count + s, offmax + s, skip,
+                                   stream);
+            }
+            JSVD_HIP_CHECK(hipMemcpyAsync(h_rot + s, rot_count + s, sizeof(int), hipMemcpyDeviceToHost, stream));
+            JSVD_HIP_CHECK(hipEventRecord(sweep_done[s & 1], stream));
+            enqueued = s + 1;
+            if (s >= 1)
+            {
+                JSVD_HIP_CHECK(hipEventSynchronize(sweep_done[(s - 1) & 1]));
+                if (h_rot[s - 1] == 0)
+                {
+                    break;
+                }
+            }
+            else if (rounds == 0)
+            {
+                break; // a single column: the first sweep performs no rotation
+            }
+        }
+        jsvd::launch_u_sigma(a_col, lda, m, n, eps, u_rm, sigma, stream);
+        jsvd::launch_transpose(v_col, ldv, v_rm, n, n, n, stream);
+        last_rot.assign(enqueued, 0);
+        std::vector<unsigned long long> off_bits(enqueued, 0);
+        JSVD_HIP_CHECK(hipMemcpyAsync(last_rot.data(), rot_count, sizeof(int) * enqueued, hipMemcpyDeviceToHost,
+                                      stream));
+        JSVD_HIP_CHECK(hipMemcpyAsync(off_bits.data(), offmax, sizeof(unsigned long long) * enqueued,
+                                      hipMemcpyDeviceToHost, stream));
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream));
+        // Sweep count as in src/kernels.cu:976-980: the first rotation-free sweep counts; without one the
+        // solver stops at max_sweeps.
+        int executed = enqueued;
+        for (int s = 0; s < enqueued; ++s)
+        {
+            if (last_rot[s] == 0)
+            {
+                executed = s + 1;
+                break;
+            }
+        }
+        last_rot.resize(executed);
+        last_off.resize(executed);
+        for (int s = 0; s < executed; ++s)
+        {
+            double x;
+            std::memcpy(&x, &off_bits[s], sizeof(double));
+            last_off[s] = x;
+        }
+        last_sweeps = executed;
+        if (sweeps_out)
+        {
+            *sweeps_out = executed;
+        }
+    }
+
+    void ensure_host_staging()
+    {
+        if (!d_a)
+        {
+            JSVD_HIP_CHECK(hipMalloc(&d_a, sizeof(double) * m * n));
+            JSVD_HIP_CHECK(hipMalloc(&d_u, sizeof(double) * m * n));
+            JSVD_HIP_CHECK(hipMalloc(&d_sigma, sizeof(double) * n));
+            JSVD_HIP_CHECK(hipMalloc(&d_v, sizeof(double) * n * n));
+        }
+    }
+
+    void solve_host(const double *a, double *u, double *sigma, double *v, int *sweeps_out)
+    {
+        ensure_host_staging();
+        JSVD_HIP_CHECK(hipMemcpyAsync(d_a, a, sizeof(double) * m * n, hipMemcpyHostToDevice, stream));
+        solve_device(d_a, d_u, d_sigma, d_v, sweeps_out);
+        JSVD_HIP_CHECK(hipMemcpyAsync(u, d_u, sizeof(double) * m * n, hipMemcpyDeviceToHost, stream));
+        JSVD_HIP_CHECK(hipMemcpyAsync(sigma, d_sigma, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+        JSVD_HIP_CHECK(hipMemcpyAsync(v, d_v, sizeof(double) * n * n, hipMemcpyDeviceToHost, stream));
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+};
+
+namespace
+{
+    void require(bool ok, const char *message)
+    {
+        if (!ok)
+        {
+            jsvd::throw_invalid(message);
+        }
+    }
+
+    bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+} // namespace
+
+extern "C" {
+
+void jsvd_config_default(jsvd_config *cfg)
+{
+    if (!cfg)
+    {
+        return;
+    }
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->epsilon = 1.0e-9;
+    cfg->max_sweeps = 128;
+    cfg->threads_per_block = 256;
+    cfg->layout_transpose_mode = JSVD_LAYOUT_AUTO;
+    cfg->layout_transpose_min_columns = 16;
+    cfg->layout_transpose_min_elements = 4096;
+    cfg->layout_transpose_auto_tune = 0;
+    cfg->layout_transpose_benchmark_repetitions = 2;
+    cfg->layout_transpose_benchmark_sweeps = 8;
+}
+
+const char *jsvd_last_error(void) { return jsvd::t_last_error.c_str(); }
+
+const char *jsvd_version(void) { return "jsvd-gfx950 0.1.0"; }
+
+int jsvd_plan_create(int64_t m, int64_t n, const jsvd_config *cfg, void *stream, jsvd_plan **plan)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan output pointer must not be null.");
+        *plan = nullptr;
+        jsvd_config c;
+        if (cfg)
+        {
+            c = *cfg;
+        }
+        else
+        {
+            jsvd_config_default(&c);
+        }
+        jsvd::validate_shape(m, n);
+        jsvd::validate_config(c);
+        auto p = std::make_unique<jsvd_plan>();
+        p->create(m, n, c, static_cast<hipStream_t>(stream));
+        *plan = p.release();
+    });
+}
+
+int jsvd_plan_destroy(jsvd_plan *plan)
+{
+    return jsvd::guarded([&] {
+        if (plan)
+        {
+            if (plan->stream)
+            {
+                (void)hipStreamSynchronize(plan->stream);
+            }
+            delete plan;
+        }
+    });
+}
+
+int jsvd_plan_solve_device(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                           int32_t *sweeps)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan must not be null.");
+        require(d_a && d_u && d_sigma && d_v, "device buffers must not be null.");
+        int s = 0;
+        plan->solve_device(d_a, d_u, d_sigma, d_v, &s);
+        if (sweeps)
+        {
+            *sweeps = s;
+        }
+    });
+}
+
+int jsvd_plan_solve_host(jsvd_plan *plan, const double *a, double *u, double *sigma, double *v, int32_t *sweeps)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan must not be null.");
+        require(a && u && sigma && v, "host buffers must not be null.");
+        int s = 0;
+        plan->solve_host(a, u, sigma, v, &s);
+        if (sweeps)
+        {
+            *sweeps = s;
+        }
+    });
+}
+
+int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *offmax, int32_t capacity)
+{
+    int written = 0;
+    const int status = jsvd::guarded([&] {
+        require(plan != nullptr, "plan must not be null.");
+        written = std::min<int>(plan->last_sweeps, std::max(capacity, 0));
+        for (int s = 0; s < written; ++s)
+        {
+            if (rotations)
+            {
+                rotations[s] = plan->last_rot[s];
+            }
+            if (offmax)
+            {
+                offmax[s] = plan->last_off[s];
+            }
+        }
+    });
+    return status == JSVD_OK ? written : -status;
+}
+
+int jsvd_plain_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, double *u, double *sigma,
+                   double *v, int32_t *sweeps)
+{
+    return jsvd::guarded([&] {
+        jsvd_config c;
+        if (cfg)
+        {
+            c = *cfg;
+        }
+        else
+        {
+            jsvd_config_default(&c);
+        }
+        jsvd::validate_shape(m, n);
+        jsvd::validate_config(c);
+        require(a && u && sigma && v, "host buffers must not be null.");
+        jsvd_plan plan;
+        plan.create(m, n, c, nullptr);
+        int s = 0;
+        plan.solve_host(a, u, sigma, v, &s);
+        if (sweeps)
+        {
+            *sweeps = s;
+        }
+    });
+}
+
+int jsvd_round_robin_pairs(int64_t n, int32_t round, int32_t *pairs_out)
+{
+    int count = 0;
+    const int status = jsvd::guarded([&] {
+        require(n >= 0 && n <= (int64_t{1} << 30), "n out of range.");
+        require(pairs_out != nullptr || n < 2, "pairs_out must not be null.");
+        if (n < 2)
+        {
+            return;
+        }
+        const int N = static_cast<int>(n + (n & 1));
+        require(round >= 0 && round < N - 1, "round out of range.");
+        for (int slot = 0; slot < N / 2; ++slot)
+        {
+            const int p = jsvd::dev::circle_at(slot, round, N);
+            const int q = jsvd::dev::circle_at(N - 1 - slot, round, N);
+            if (p < n && q < n)
+            {
+                pairs_out[2 * count] = p;
+                pairs_out[2 * count + 1] = q;
+                ++count;
+            }
+        }
+    });
+    return status == JSVD_OK ? count : -status;
+}
+
+int jsvd_transpose_f64(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols,
+                       void *stream)
+{
+    return jsvd::guarded([&] {
+        require(rows >= 0 && cols >= 0, "rows and cols must be non-negative.");
+        require(lds >= cols && ldd >= rows, "leading dimensions too small.");
+        require((src && dst) || rows == 0 || cols == 0, "buffers must not be null.");
+        jsvd::launch_transpose(src, lds, dst, ldd, rows, cols, static_cast<hipStream_t>(stream));
+    });
+}
+
+int jsvd_pair_gram_f64(const double *a, int64_t lda, int64_t m, const int32_t *pairs, int64_t count, double *alpha,
+                       double *beta, double *gamma, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(m >= 0 && lda >= m && count >= 0, "invalid sizes.");
+        require(count == 0 || (a && pairs && alpha && beta && gamma), "buffers must not be null.");
+        jsvd::launch_pair_gram(a, lda, m, pairs, count, alpha, beta, gamma, static_cast<hipStream_t>(stream));
+    });
+}
+
+int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t round,
+                   double epsilon, int32_t *rot_count, uint64_t *offmax, const int32_t *skip_if_zero, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(m > 0 && n > 0 && m >= n, "invalid shape (need m >= n > 0).");
+        require(lda >= m && lda % 2 == 0, "lda must be even and >= m.");
+        require(ldv >= n && ldv % 2 == 0, "ldv must be even and >= n.");
+        require(a && v && aligned16(a) && aligned16(v), "a and v must be non-null 16-byte aligned device pointers.");
+        const int64_t N = n + (n & 1);
+        require(n < 2 || (round >= 0 && round < N - 1), "round out of range.");
+        require(epsilon > 0.0, "epsilon must be positive.");
+        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n);
+        jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
+                           reinterpret_cast<unsigned long long *>(offmax), skip_if_zero,
+                           static_cast<hipStream_t>(stream));
+    });
+}
+
+int jsvd_rotation_params_f64(const double *alpha, const double *beta, const double *gamma, int64_t count,
+                             double epsilon, double *c, double *s, int32_t *rotate, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(count >= 0, "count must be non-negative.");
+        require(count == 0 || (alpha && beta && gamma && c && s && rotate), "buffers must not be null.");
+        jsvd::launch_rotation_params(alpha, beta, gamma, count, epsilon, c, s, rotate,
+                                     static_cast<hipStream_t>(stream));
+    });
+}
+
+int jsvd_offnorm_f64(const double *a, int64_t lda, int64_t m, int64_t n, double *out, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(m > 0 && n > 0 && lda >= m, "invalid sizes.");
+        require(a && out, "buffers must not be null.");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        double *workspace = nullptr;
+        JSVD_HIP_CHECK(hipMallocAsync(&workspace, sizeof(double) * jsvd::offnorm_workspace_doubles(n), s));
+        jsvd::launch_offnorm(a, lda, m, n, workspace, out, s);
+        JSVD_HIP_CHECK(hipFreeAsync(workspace, s));
+    });
+}
+
+int jsvd_u_sigma_f64(const double *a, int64_t lda, int64_t m, int64_t n, double epsilon, double *u, double *sigma,
+                     void *stream)
+{
+    return jsvd::guarded([&] {
+        require(m > 0 && n > 0 && lda >= m, "invalid sizes.");
+        require(a && u && sigma, "buffers must not be null.");
+        jsvd::launch_u_sigma(a, lda, m, n, epsilon, u, sigma, static_cast<hipStream_t>(stream));
+    });
+}
+
+} // extern "C"

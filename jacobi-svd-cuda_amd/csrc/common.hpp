@@ -1,0 +1,165 @@
+// common.hpp -- shared helpers of libjsvd: status/error plumbing for the C ABI, HIP error checks,
+// device reductions and the closed-form round-robin schedule.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "jsvd.h"
+
+namespace jsvd
+{
+    // Exception carrying a C-ABI status code.  Thrown inside the library, caught at the ABI edge and
+    // turned into (status, thread-local message).
+    class Error : public std::runtime_error
+    {
+    public:
+        Error(int status, const std::string &message) : std::runtime_error(message), status_(status) {}
+        [[nodiscard]] int status() const noexcept { return status_; }
+
+    private:
+        int status_;
+    };
+
+    [[noreturn]] inline void throw_invalid(const std::string &message)
+    {
+        throw Error(JSVD_ERR_INVALID_ARGUMENT, message);
+    }
+
+    // Message shape follows the reference's CUDA check (src/kernels.cu:29-40), with HIP names.
+    [[noreturn]] void throw_hip(hipError_t status, const char *expression, const char *file, int line);
+
+    void set_last_error(const std::string &message);
+    void clear_last_error();
+
+    // Runs `body` and converts any exception into a status code plus the thread-local message.
+    template <typename F>
+    int guarded(F &&body) noexcept
+    {
+        try
+        {
+            clear_last_error();
+            body();
+            return JSVD_OK;
+        }
+        catch (const Error &e)
+        {
+            set_last_error(e.what());
+            return e.status();
+        }
+        catch (const std::invalid_argument &e)
+        {
+            set_last_error(e.what());
+            return JSVD_ERR_INVALID_ARGUMENT;
+        }
+        catch (const std::exception &e)
+        {
+            set_last_error(e.what());
+            return JSVD_ERR_INTERNAL;
+        }
+        catch (...)
+        {
+            set_last_error("unknown exception");
+            return JSVD_ERR_INTERNAL;
+        }
+    }
+
+    // Validation of the solver configuration, restating src/kernels.cu:338-356 and 807-833.
+    void validate_config(const jsvd_config &cfg);
+    void validate_shape(int64_t m, int64_t n);
+    // Reference normalisation of threads_per_block (src/kernels.cu:685-690).
+    int normalize_threads_per_block(int raw);
+
+    inline int64_t round_up(int64_t x, int64_t k) { return (x + k - 1) / k * k; }
+} // namespace jsvd
+
+#define JSVD_HIP_CHECK(EXPR)                                                                  \
+    do                                                                                        \
+    {                                                                                         \
+        const hipError_t jsvd_status_ = (EXPR);                                               \
+        if (jsvd_status_ != hipSuccess)                                                       \
+        {                                                                                     \
+            ::jsvd::throw_hip(jsvd_status_, #EXPR, __FILE__, __LINE__);                        \
+        }                                                                                     \
+    } while (0)
+
+namespace jsvd::dev
+{
+    // Position `slot` of the round-robin circle at round `r` for an even circle of size N.
+    // The circle starts as 0..N-1 (value N-1 is the dummy when the column count is odd) and after every
+    // round positions 1..N-1 rotate right by one, so position i >= 1 holds 1 + ((i-1-r) mod (N-1)).
+    // This is the closed form of the host schedule of src/kernels.cu:637-678.
+    __host__ __device__ inline int circle_at(int slot, int r, int N)
+    {
+        if (slot == 0)
+        {
+            return 0;
+        }
+        const int period = N - 1;
+        int k = (slot - 1 - r) % period;
+        if (k < 0)
+        {
+            k += period;
+        }
+        return 1 + k;
+    }
+
+    // Rotation parameters and skip test in FP64 (src/kernels.cu:500-511).  Contraction is disabled so
+    // the result is bit-identical to an IEEE evaluation of the same formula on the host.
+    __host__ __device__ inline bool rotation_params(double app, double aqq, double apq, double eps, double &c,
+                                                    double &s)
+    {
+#pragma clang fp contract(off)
+        const double rhs = eps * sqrt(fmax(app, 0.0) * fmax(aqq, 0.0));
+        if (fabs(apq) > rhs && fabs(apq) > 1.0e-300)
+        {
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double root = sqrt(1.0 + tau * tau);
+            const double t = (tau >= 0.0) ? (1.0 / (tau + root)) : (-1.0 / (-tau + root));
+            const double cosine = 1.0 / sqrt(1.0 + t * t);
+            c = cosine;
+            s = t * cosine;
+            return true;
+        }
+        c = 1.0;
+        s = 0.0;
+        return false;
+    }
+
+    // Off-orthogonality of a pair, |gamma| / sqrt(alpha beta), 0 when a column is zero.
+    __host__ __device__ inline double pair_cosine(double app, double aqq, double apq)
+    {
+        const double denom = sqrt(fmax(app, 0.0) * fmax(aqq, 0.0));
+        return denom > 0.0 ? fabs(apq) / denom : 0.0;
+    }
+
+    // Butterfly sum over the 64 lanes of a wavefront; every lane ends with the total.
+    __device__ inline double wave_sum(double x)
+    {
+#pragma unroll
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            x += __shfl_xor(x, offset, 64);
+        }
+        return x;
+    }
+
+    __device__ inline double wave_max(double x)
+    {
+#pragma unroll
+        for (int offset = 32; offset > 0; offset >>= 1)
+        {
+            x = fmax(x, __shfl_xor(x, offset, 64));
+        }
+        return x;
+    }
+
+    // atomicMax of a non-negative double through its bit pattern (IEEE order == unsigned order for x >= 0).
+    __device__ inline void atomic_max_nonneg(unsigned long long *slot, double x)
+    {
+        atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(x)));
+    }
+} // namespace jsvd::dev

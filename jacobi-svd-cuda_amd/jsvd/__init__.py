@@ -1,0 +1,297 @@
+"""Python host binding of libjsvd.so (ctypes over the C ABI declared in include/jsvd.h).
+
+This mirrors the reference's C++ entry points for scripting, tests and benchmarks:
+
+* ``one_sided_jacobi_svd(a, config)`` -- jacobi::svd::one_sided_jacobi_svd
+  (include/kernels.hpp:295-298): host row-major float64 in, ``JacobiSvdResult`` out.
+* ``Plan`` -- a reusable device workspace for one shape (device-resident solves on torch tensors).
+* kernel-level wrappers (``transpose``, ``pair_gram``, ``round_``, ``rotation_params``, ``offnorm``,
+  ``u_sigma``) operating on torch device tensors, enqueued on torch's current stream.
+
+The library is required: importing works without it, but every call raises ``RuntimeError`` when
+lib/libjsvd.so has not been built.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+import re
+from typing import Optional
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libjsvd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG), "include", "jsvd.h")
+
+JSVD_OK = 0
+JSVD_ERR_INVALID_ARGUMENT = 1
+JSVD_ERR_DEVICE = 2
+JSVD_ERR_INTERNAL = 3
+
+LAYOUT_AUTO, LAYOUT_FORCE_ENABLE, LAYOUT_FORCE_DISABLE = 0, 1, 2
+
+
+class CudaError(RuntimeError):
+    """Device failure (the reference's jacobi::svd::CudaError, include/kernels.hpp:65-73)."""
+
+
+class JsvdConfig(ctypes.Structure):
+    """jsvd_config, field-for-field JacobiSvdConfig (include/kernels.hpp:78-124)."""
+    _fields_ = [
+        ("epsilon", ctypes.c_double),
+        ("max_sweeps", ctypes.c_int32),
+        ("threads_per_block", ctypes.c_int32),
+        ("layout_transpose_mode", ctypes.c_int32),
+        ("layout_transpose_min_columns", ctypes.c_int32),
+        ("layout_transpose_min_elements", ctypes.c_uint64),
+        ("layout_transpose_auto_tune", ctypes.c_int32),
+        ("layout_transpose_benchmark_repetitions", ctypes.c_int32),
+        ("layout_transpose_benchmark_sweeps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+@dataclasses.dataclass
+class JacobiSvdConfig:
+    """Python mirror of jacobi::svd::JacobiSvdConfig with the reference defaults (kernels.hpp:83-123)."""
+    epsilon: float = 1.0e-9
+    max_sweeps: int = 128
+    threads_per_block: int = 256
+    layout_transpose_mode: int = LAYOUT_AUTO
+    layout_transpose_min_columns: int = 16
+    layout_transpose_min_elements: int = 4096
+    layout_transpose_auto_tune: bool = False
+    layout_transpose_benchmark_repetitions: int = 2
+    layout_transpose_benchmark_sweeps: int = 8
+
+    def to_c(self) -> JsvdConfig:
+        c = JsvdConfig()
+        c.epsilon = self.epsilon
+        c.max_sweeps = self.max_sweeps
+        c.threads_per_block = self.threads_per_block
+        c.layout_transpose_mode = self.layout_transpose_mode
+        c.layout_transpose_min_columns = self.layout_transpose_min_columns
+        c.layout_transpose_min_elements = self.layout_transpose_min_elements
+        c.layout_transpose_auto_tune = int(bool(self.layout_transpose_auto_tune))
+        c.layout_transpose_benchmark_repetitions = self.layout_transpose_benchmark_repetitions
+        c.layout_transpose_benchmark_sweeps = self.layout_transpose_benchmark_sweeps
+        return c
+
+
+@dataclasses.dataclass
+class JacobiSvdResult:
+    """jacobi::svd::JacobiSvdResult (include/kernels.hpp:249-280): row-major U (m x n), sigma (n), V (n x n)."""
+    rows: int
+    columns: int
+    u: np.ndarray
+    sigma: np.ndarray
+    v: np.ndarray
+    sweeps: int
+
+
+_lib = None
+_dptr = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def declared_symbols() -> list[str]:
+    """Every function name declared in include/jsvd.h."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(jsvd_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libjsvd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.jsvd_config_default.argtypes = [ctypes.POINTER(JsvdConfig)]
+        L.jsvd_config_default.restype = None
+        L.jsvd_last_error.restype = ctypes.c_char_p
+        L.jsvd_version.restype = ctypes.c_char_p
+        L.jsvd_plain_f64.argtypes = [_vp, _i64, _i64, ctypes.POINTER(JsvdConfig), _vp, _vp, _vp, _i32p]
+        L.jsvd_plan_create.argtypes = [_i64, _i64, ctypes.POINTER(JsvdConfig), _vp, ctypes.POINTER(_vp)]
+        L.jsvd_plan_destroy.argtypes = [_vp]
+        L.jsvd_plan_solve_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _i32p]
+        L.jsvd_plan_solve_host.argtypes = [_vp, _vp, _vp, _vp, _vp, _i32p]
+        L.jsvd_plan_sweep_stats.argtypes = [_vp, _i32p, _dptr, ctypes.c_int32]
+        L.jsvd_transpose_f64.argtypes = [_vp, _i64, _vp, _i64, _i64, _i64, _vp]
+        L.jsvd_pair_gram_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _vp, _vp]
+        L.jsvd_round_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_double, _vp, _vp,
+                                     _vp, _vp]
+        L.jsvd_rotation_params_f64.argtypes = [_vp, _vp, _vp, _i64, ctypes.c_double, _vp, _vp, _vp, _vp]
+        L.jsvd_round_robin_pairs.argtypes = [_i64, ctypes.c_int32, _i32p]
+        L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
+        L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
+        for name in declared_symbols():
+            if name not in ("jsvd_config_default", "jsvd_last_error", "jsvd_version"):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return (lib().jsvd_last_error() or b"").decode()
+
+
+def check(status: int) -> None:
+    if status == JSVD_OK:
+        return
+    msg = last_error()
+    if status == JSVD_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == JSVD_ERR_DEVICE:
+        raise CudaError(msg)
+    raise RuntimeError(msg)
+
+
+def default_config() -> JsvdConfig:
+    c = JsvdConfig()
+    lib().jsvd_config_default(ctypes.byref(c))
+    return c
+
+
+def _as_c_config(config) -> JsvdConfig:
+    if config is None:
+        return default_config()
+    if isinstance(config, JsvdConfig):
+        return config
+    return config.to_c()
+
+
+def one_sided_jacobi_svd(a, config: Optional[JacobiSvdConfig] = None) -> JacobiSvdResult:
+    """Host-array solve (jacobi::svd::one_sided_jacobi_svd).  a: row-major float64 m x n with m >= n."""
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if arr.ndim != 2:
+        raise ValueError("Input buffer size mismatch.")
+    m, n = arr.shape
+    u = np.zeros((m, n))
+    s = np.zeros(n)
+    v = np.zeros((n, n))
+    sweeps = ctypes.c_int32()
+    cfg = _as_c_config(config)
+    check(lib().jsvd_plain_f64(arr.ctypes.data, m, n, ctypes.byref(cfg), u.ctypes.data, s.ctypes.data,
+                               v.ctypes.data, ctypes.byref(sweeps)))
+    return JacobiSvdResult(m, n, u, s, v, sweeps.value)
+
+
+def round_robin_pairs(n: int, rnd: int) -> list[tuple[int, int]]:
+    """The closed-form schedule the kernels evaluate, computed on the host (jsvd_round_robin_pairs)."""
+    buf = np.zeros(2 * ((n + 1) // 2 + 1), dtype=np.int32)
+    k = lib().jsvd_round_robin_pairs(n, rnd, buf.ctypes.data_as(_i32p))
+    if k < 0:
+        check(-k)
+    return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(k)]
+
+
+def _stream_handle(stream=None) -> int:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+class Plan:
+    """Reusable solver workspace for one shape (jsvd_plan_*)."""
+
+    def __init__(self, m: int, n: int, config=None, stream=None):
+        self.m, self.n = int(m), int(n)
+        self._cfg = _as_c_config(config)
+        handle = _vp()
+        s = _stream_handle(stream) if stream is not None else None
+        check(lib().jsvd_plan_create(self.m, self.n, ctypes.byref(self._cfg), s, ctypes.byref(handle)))
+        self._h = handle
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().jsvd_plan_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_device(self, d_a, d_u, d_sigma, d_v) -> int:
+        """Device-resident solve on torch float64 tensors (row-major, contiguous).
+
+        The plan runs on its own stream, so pending torch work on the current stream (the producers of
+        d_a, allocations of the outputs) is waited for first; the call returns after the solve completed.
+        """
+        import torch
+        for t, shape in ((d_a, (self.m, self.n)), (d_u, (self.m, self.n)), (d_sigma, (self.n,)),
+                         (d_v, (self.n, self.n))):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+                raise ValueError(f"expected a contiguous cuda float64 tensor of shape {shape}")
+        torch.cuda.current_stream().synchronize()
+        sweeps = ctypes.c_int32()
+        check(lib().jsvd_plan_solve_device(self._h, _ptr(d_a), _ptr(d_u), _ptr(d_sigma), _ptr(d_v),
+                                           ctypes.byref(sweeps)))
+        return sweeps.value
+
+    def solve_host(self, a: np.ndarray) -> JacobiSvdResult:
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        u = np.zeros((self.m, self.n))
+        s = np.zeros(self.n)
+        v = np.zeros((self.n, self.n))
+        sweeps = ctypes.c_int32()
+        check(lib().jsvd_plan_solve_host(self._h, arr.ctypes.data, u.ctypes.data, s.ctypes.data, v.ctypes.data,
+                                         ctypes.byref(sweeps)))
+        return JacobiSvdResult(self.m, self.n, u, s, v, sweeps.value)
+
+    def sweep_stats(self, capacity: int = 1024):
+        rot = np.zeros(capacity, dtype=np.int32)
+        off = np.zeros(capacity)
+        k = lib().jsvd_plan_sweep_stats(self._h, rot.ctypes.data_as(_i32p), off.ctypes.data_as(_dptr), capacity)
+        if k < 0:
+            check(-k)
+        return rot[:k].copy(), off[:k].copy()
+
+
+# ------------------------------------------------------------------------------------------------
+# kernel-level wrappers (torch device tensors; enqueued on the current torch stream)
+# ------------------------------------------------------------------------------------------------
+
+def transpose(src, dst, rows: int, cols: int, lds: int, ldd: int, stream=None) -> None:
+    check(lib().jsvd_transpose_f64(_ptr(src), lds, _ptr(dst), ldd, rows, cols, _stream_handle(stream)))
+
+
+def pair_gram(a_col, lda: int, m: int, pairs, alpha, beta, gamma, stream=None) -> None:
+    check(lib().jsvd_pair_gram_f64(_ptr(a_col), lda, m, _ptr(pairs), pairs.numel() // 2, _ptr(alpha), _ptr(beta),
+                                   _ptr(gamma), _stream_handle(stream)))
+
+
+def round_(a_col, lda: int, m: int, v_col, ldv: int, n: int, rnd: int, eps: float, rot_count=None, offmax=None,
+           skip_if_zero=None, stream=None) -> None:
+    check(lib().jsvd_round_f64(_ptr(a_col), lda, m, _ptr(v_col), ldv, n, rnd, eps,
+                               _ptr(rot_count) if rot_count is not None else None,
+                               _ptr(offmax) if offmax is not None else None,
+                               _ptr(skip_if_zero) if skip_if_zero is not None else None,
+                               _stream_handle(stream)))
+
+
+def rotation_params(alpha, beta, gamma, eps: float, c, s, rotate, stream=None) -> None:
+    check(lib().jsvd_rotation_params_f64(_ptr(alpha), _ptr(beta), _ptr(gamma), alpha.numel(), eps, _ptr(c), _ptr(s),
+                                         _ptr(rotate), _stream_handle(stream)))
+
+
+def offnorm(a_col, lda: int, m: int, n: int, out, stream=None) -> None:
+    check(lib().jsvd_offnorm_f64(_ptr(a_col), lda, m, n, _ptr(out), _stream_handle(stream)))
+
+
+def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) -> None:
+    check(lib().jsvd_u_sigma_f64(_ptr(a_col), lda, m, n, eps, _ptr(u), _ptr(sigma), _stream_handle(stream)))

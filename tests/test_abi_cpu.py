@@ -1,0 +1,58 @@
+"""Host-side checks of libjsvd.so that need no GPU: it loads, exports every symbol include/jsvd.h
+declares, carries the reference defaults, validates inputs like the reference (src/kernels.cu:807-833,
+338-356) before touching the device, and its closed-form schedule equals the oracle's schedule."""
+import numpy as np
+import pytest
+
+import jsvd
+import oracle
+
+
+def test_library_exports_every_declared_symbol():
+    lib = jsvd.lib()
+    names = jsvd.declared_symbols()
+    assert "jsvd_plain_f64" in names and "jsvd_round_f64" in names
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_default_config_matches_reference():
+    c = jsvd.default_config()
+    assert c.epsilon == 1e-9 and c.max_sweeps == 128 and c.threads_per_block == 256
+    assert c.layout_transpose_mode == jsvd.LAYOUT_AUTO
+    assert c.layout_transpose_min_columns == 16 and c.layout_transpose_min_elements == 4096
+    assert c.layout_transpose_auto_tune == 0
+    assert c.layout_transpose_benchmark_repetitions == 2 and c.layout_transpose_benchmark_sweeps == 8
+
+
+@pytest.mark.parametrize("shape,cfg,msg", [
+    ((3, 4), {}, "requires rows >= columns"),
+    ((0, 0), {}, "shape must be non-zero"),
+    ((4, 3), {"epsilon": 0.0}, "epsilon must be positive"),
+    ((4, 3), {"epsilon": -1.0}, "epsilon must be positive"),
+    ((4, 3), {"max_sweeps": 0}, "max_sweeps must be positive"),
+    ((4, 3), {"layout_transpose_min_columns": 0}, "layout_transpose_min_columns must be positive"),
+    ((4, 3), {"layout_transpose_min_elements": 0}, "layout_transpose_min_elements must be positive"),
+    ((4, 3), {"layout_transpose_benchmark_repetitions": 0}, "repetitions must be positive"),
+    ((4, 3), {"layout_transpose_benchmark_sweeps": 0}, "benchmark_sweeps must be positive"),
+])
+def test_invalid_arguments_raise_before_device(shape, cfg, msg):
+    a = np.zeros(shape)
+    with pytest.raises(ValueError, match=msg):
+        jsvd.one_sided_jacobi_svd(a, jsvd.JacobiSvdConfig(**cfg))
+
+
+@pytest.mark.parametrize("n", list(range(1, 34)) + [64, 99, 128, 255, 256, 1023, 1024])
+def test_closed_form_schedule_equals_oracle(n):
+    ref = oracle.round_robin(n)
+    if n < 2:
+        assert ref == []
+        assert jsvd.round_robin_pairs(n, 0) == []
+        return
+    for r, pairs in enumerate(ref):
+        assert jsvd.round_robin_pairs(n, r) == pairs
+
+
+def test_schedule_round_out_of_range():
+    with pytest.raises(ValueError):
+        jsvd.round_robin_pairs(8, 7)
