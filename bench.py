@@ -2,7 +2,7 @@
 
 Default workload (one step = one full SVD solve, input resident in HBM, outputs U, sigma, V in HBM):
   tall_geometric 4096 x 1024, A = Q1 diag(sigma) Q2^T with sigma geometric from 1 to 1e-8,
-  plain FP64 cyclic one-sided Jacobi, eps 1e-15, at most 30 sweeps.
+  plain FP64 cyclic one-sided Jacobi, eps 1e-14, at most 60 sweeps.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel (the fused round kernel,
 timed with HIP events on the stream it runs on) and a CPU baseline (the oracle's C restatement of the
@@ -34,7 +34,9 @@ def make_workload(name: str):
     from jsvd import workloads
     if name == "tall":
         a, sigma = workloads.tall_geometric()
-        return a, sigma, 1e-15, 30, "tall_geometric_4096x1024_fp64"
+        # eps 1e-14 >= sqrt(m) * unit roundoff (7.1e-15 at m = 4096): below it the dot-product rounding
+        # noise of the cosine test never settles; the reference algorithm needs ~40 sweeps on this matrix.
+        return a, sigma, 1e-14, 60, "tall_geometric_4096x1024_fp64"
     if name == "plain":
         a = workloads.plain_gaussian()
         return a, None, 1e-15, 30, "plain_gaussian_512x256_fp64"
@@ -70,6 +72,38 @@ def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
     # reads + writes its two V columns.
     algo_bytes = 8 * (pairs * 2 * m + rotated * (2 * m + 4 * n))
     return 1000.0 * total_ms / launches, algo_bytes / launches, rotated, pairs
+
+
+def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
+    """Round kernel on an HBM-resident shape (A 512 MiB + V 128 MiB, above the 256 MiB MALL): a column-graded
+    N(0,1) matrix generated on the device; the first rounds rotate every pair."""
+    lda, ldv = m, n
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    acol = torch.randn((n, lda), dtype=torch.float64, device="cuda", generator=gen)
+    acol *= torch.pow(10.0, -12.0 * torch.arange(n, device="cuda", dtype=torch.float64) / (n - 1)).unsqueeze(1)
+    vcol = torch.eye(n, dtype=torch.float64, device="cuda")
+    rot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    jsvd.round_(acol, lda, m, vcol, ldv, n, 0, 1e-15, stream=stream)   # warm the code path
+    torch.cuda.synchronize()
+    for r in range(launches):
+        starts[r].record(stream)
+        jsvd.round_(acol, lda, m, vcol, ldv, n, r + 1, 1e-15, rot_count=rot, stream=stream)
+        ends[r].record(stream)
+    torch.cuda.synchronize()
+    total_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends))
+    rotated = int(rot.item())
+    pairs = launches * (n // 2)
+    algo_bytes = 8 * (pairs * 2 * m + rotated * (2 * m + 4 * n)) / launches
+    avg_us = 1000.0 * total_ms / launches
+    achieved = algo_bytes / (avg_us * 1e-6) / 1e9
+    del acol, vcol
+    return {"shape": f"{m}x{n} column-graded", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "avg_launch_us": avg_us, "algorithmic_bytes_per_launch": algo_bytes,
+            "rotated_pairs": rotated, "pairs": pairs}
 
 
 def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
@@ -175,6 +209,9 @@ def main() -> None:
         "rotated_pairs": rotated,
         "pairs": pairs,
     }
+    log("[bench] round-kernel roofline on the HBM-resident 16384x4096 shape")
+    roofline_hbm = round_kernel_hbm_roofline(torch, jsvd)
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -200,6 +237,7 @@ def main() -> None:
                    "path": "plain_f64"},
         "accuracy": acc,
         "roofline": roofline,
+        "roofline_hbm_resident": roofline_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

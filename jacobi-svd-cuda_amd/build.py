@@ -81,7 +81,27 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link of libjsvd.so failed")
     shutil.move(tmp, LIB)
+    build_cpp_test(verbose)
     return LIB
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+CPP_TEST_BIN = os.path.join(LIBDIR, "test_dropin")
+
+
+def build_cpp_test(verbose: bool = False) -> str:
+    """The C++ drop-in test driver: plain g++ against include/*.hpp, linked to libjsvd.so."""
+    tmp = CPP_TEST_BIN + ".tmp"
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC, "-o", tmp,
+           "-L", LIBDIR, "-ljsvd", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("build of the C++ drop-in test failed")
+    shutil.move(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
 
 
 def main() -> None:

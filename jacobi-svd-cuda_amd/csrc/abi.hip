@@ -78,6 +78,22 @@ namespace jsvd
         }
     }
 
+    // The reference's layout policy (src/kernels.cu:365-380) decides between its strided and its transposed
+    // path; here it decides between the two launch shapes of short columns.
+    bool use_wide_shape(int64_t m, int64_t n, const jsvd_config &cfg)
+    {
+        if (cfg.layout_transpose_mode == JSVD_LAYOUT_FORCE_ENABLE)
+        {
+            return true;
+        }
+        if (cfg.layout_transpose_mode == JSVD_LAYOUT_FORCE_DISABLE)
+        {
+            return false;
+        }
+        return n >= cfg.layout_transpose_min_columns &&
+               static_cast<uint64_t>(m) * static_cast<uint64_t>(n) >= cfg.layout_transpose_min_elements;
+    }
+
     int normalize_threads_per_block(int raw)
     {
         const int clamped = std::clamp(raw, 32, 1024);
@@ -144,7 +160,7 @@ struct jsvd_plan
         cfg = config;
         lda = jsvd::round_up(m, 2);
         ldv = jsvd::round_up(n, 2);
-        shape = jsvd::choose_round_shape(m, n);
+        shape = jsvd::choose_round_shape(m, n, jsvd::use_wide_shape(m, n, cfg));
         if (user_stream)
         {
             stream = user_stream;
@@ -476,7 +492,7 @@ int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, in
         const int64_t N = n + (n & 1);
         require(n < 2 || (round >= 0 && round < N - 1), "round out of range.");
         require(epsilon > 0.0, "epsilon must be positive.");
-        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n);
+        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n, false);
         jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
                            reinterpret_cast<unsigned long long *>(offmax), skip_if_zero,
                            static_cast<hipStream_t>(stream));

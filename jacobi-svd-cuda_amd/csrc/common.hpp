@@ -72,6 +72,7 @@ namespace jsvd
     void validate_shape(int64_t m, int64_t n);
     // Reference normalisation of threads_per_block (src/kernels.cu:685-690).
     int normalize_threads_per_block(int raw);
+    bool use_wide_shape(int64_t m, int64_t n, const jsvd_config &cfg);
 
     inline int64_t round_up(int64_t x, int64_t k) { return (x + k - 1) / k * k; }
 } // namespace jsvd

@@ -526,12 +526,12 @@ namespace jsvd
         }
     } // namespace
 
-    RoundShape choose_round_shape(int64_t m, int64_t n)
+    RoundShape choose_round_shape(int64_t m, int64_t n, bool wide)
     {
         RoundShape shape{};
         const int64_t m2 = (m + 1) / 2;
         const int64_t n2 = (n + 1) / 2;
-        if (m2 <= 64 * 8)
+        if (m2 <= 64 * 8 && !wide)
         {
             shape.threads = 64;
         }

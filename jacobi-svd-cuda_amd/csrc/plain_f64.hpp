@@ -15,7 +15,9 @@ namespace jsvd
         int ev = 1;
     };
 
-    RoundShape choose_round_shape(int64_t m, int64_t n);
+    // `wide` selects a 256-thread workgroup per pair where one wavefront per pair would also fit
+    // (m <= 1024); longer columns always take 256- or 1024-thread workgroups.
+    RoundShape choose_round_shape(int64_t m, int64_t n, bool wide);
 
     // One fused round (Gram + rotation + apply to A and V) over the N/2 slots of round `round`.
     void launch_round(const RoundShape &shape, double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n,
