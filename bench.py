@@ -131,6 +131,52 @@ def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
     }
 
 
+def time_path(torch, jsvd, path, a_host, sigma_true, cfg, steps, warmup):
+    """Warmup + `steps` timed device-resident solves through one plan; returns the report of the path."""
+    m, n = a_host.shape
+    d_a = torch.from_numpy(a_host).to("cuda")
+    d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
+    d_s = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    plan = jsvd.Plan(m, n, cfg, path=path)
+    for _ in range(warmup):
+        plan.solve_device(d_a, d_u, d_s, d_v)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        plan.solve_device(d_a, d_u, d_s, d_v)   # returns after the solve completed
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    info = plan.last_info().as_dict()
+    plan.close()
+    sig = d_s.cpu().numpy()
+    u = d_u.cpu().numpy()
+    v = d_v.cpu().numpy()
+    acc = {
+        "rho_A": float(np.linalg.norm(a_host - (u * sig) @ v.T) / max(1.0, np.linalg.norm(a_host))),
+        "rho_U": float(np.linalg.norm(u.T @ u - np.eye(n))),
+        "rho_V": float(np.linalg.norm(v.T @ v - np.eye(n))),
+    }
+    if sigma_true is not None:
+        s_sorted = np.sort(sig)[::-1]
+        acc["sigma_err_normwise"] = float(np.max(np.abs(s_sorted - sigma_true)) / sigma_true[0])
+        acc["sigma_relerr_max"] = float(np.max(np.abs(s_sorted - sigma_true) / sigma_true))
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "path": jsvd.PATH_NAMES[path],
+        "solves_per_s": steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "info": info,
+        "accuracy": acc,
+        "sigma": sig,
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +184,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tall", choices=["tall", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -153,51 +200,24 @@ def main() -> None:
     m, n = a_host.shape
     cfg = jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=max_sweeps)
     torch.cuda.init()
-    d_a = torch.from_numpy(a_host).to("cuda")
-    d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
-    d_s = torch.empty(n, dtype=torch.float64, device="cuda")
-    d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
-    plan = jsvd.Plan(m, n, cfg)
 
-    log(f"[bench] warmup {args.warmup}")
-    for _ in range(args.warmup):
-        plan.solve_device(d_a, d_u, d_s, d_v)
-    torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} steps")
-    lat = []
-    sweeps = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        sweeps = plan.solve_device(d_a, d_u, d_s, d_v)   # returns after the solve completed
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    value = args.steps / elapsed
-    lat_ms = np.array(lat) * 1e3
-
-    sig = d_s.cpu().numpy()
-    u = d_u.cpu().numpy()
-    v = d_v.cpu().numpy()
-    acc = {
-        "sweeps": int(sweeps),
-        "rho_A": float(np.linalg.norm(a_host - (u * sig) @ v.T) / max(1.0, np.linalg.norm(a_host))),
-        "rho_U": float(np.linalg.norm(u.T @ u - np.eye(n))),
-        "rho_V": float(np.linalg.norm(v.T @ v - np.eye(n))),
-    }
-    if sigma_true is not None:
-        s_sorted = np.sort(sig)[::-1]
-        acc["sigma_err_normwise"] = float(np.max(np.abs(s_sorted - sigma_true)) / sigma_true[0])
-        acc["sigma_relerr_max"] = float(np.max(np.abs(s_sorted - sigma_true) / sigma_true))
-    log(f"[bench] accuracy {acc}")
+    log(f"[bench] mixed path: warmup {args.warmup}, {args.steps} timed steps")
+    mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, args.steps, args.warmup)
+    log(f"[bench] mixed: {mixed['solves_per_s']:.2f} solves/s info={mixed['info']} acc={mixed['accuracy']}")
+    plain = None
+    if not args.no_plain:
+        log(f"[bench] plain path: warmup {args.warmup}, {args.steps} timed steps")
+        plain = time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, a_host, sigma_true, cfg, args.steps, args.warmup)
+        log(f"[bench] plain: {plain['solves_per_s']:.2f} solves/s info={plain['info']} acc={plain['accuracy']}")
+        # sigma agreement of the two GPU paths (sorted)
+        d = np.abs(np.sort(mixed["sigma"]) - np.sort(plain["sigma"]))
+        mixed["accuracy"]["sigma_vs_plain_normwise"] = float(d.max() / np.max(plain["sigma"]))
 
     log("[bench] round-kernel roofline")
     avg_us, bytes_per_launch, rotated, pairs = round_kernel_roofline(torch, jsvd, a_host, eps)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     roofline = {
-        "kernel": "round_kernel (fused Gram + rotation + Givens apply, one launch per round)",
+        "kernel": "round_kernel (plain FP64 fused Gram + rotation + Givens apply, one launch per round)",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
@@ -216,32 +236,37 @@ def main() -> None:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline(a_host, eps, sweeps, args.cpu_sample_sweeps)
+        gpu_plain_sweeps = plain["info"]["sweeps_fp64"] if plain else max_sweeps
+        cpu = cpu_baseline(a_host, eps, gpu_plain_sweeps, args.cpu_sample_sweeps)
 
+    for rep in (mixed, plain):
+        if rep is not None:
+            rep.pop("sigma", None)
     out = {
-        "metric": "SVD solves/s + p99 latency (plain FP64 one-sided Jacobi)",
-        "value": value,
+        "metric": "SVD solves/s + p99 latency, FP32-MFMA+FP64-refine vs plain FP64 at sigma relerr<=1e-12",
+        "value": mixed["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "ms_per_step": mixed["ms_per_step"],
+        "p50_ms": mixed["p50_ms"],
+        "p99_ms": mixed["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": wname, "m": m, "n": n, "epsilon": eps, "max_sweeps": max_sweeps,
-                   "path": "plain_f64"},
-        "accuracy": acc,
+                   "path": "mixed_f32_f64"},
+        "mixed": mixed,
+        "plain_f64": plain,
+        "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
         "roofline": roofline,
         "roofline_hbm_resident": roofline_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    plan.close()
 
 
 if __name__ == "__main__":

@@ -103,6 +103,53 @@ int jsvd_plan_solve_host(jsvd_plan *plan, const double *a, double *u, double *si
  */
 int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *offmax, int32_t capacity);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Block and mixed-precision paths.  No reference counterpart: they add precision options beside the
+ * reference's plain FP64 solver (one_sided_jacobi_svd, include/kernels.hpp:295-298) and keep its output
+ * contract (row-major U, sigma in Jacobi column order, row-major V, sweep counts, the same convergence
+ * rule "stop after a sweep without rotations", applied to block sweeps).
+ * ------------------------------------------------------------------------------------------------- */
+#define JSVD_PATH_PLAIN_F64 0 /* column-pair cyclic Jacobi, the reference algorithm */
+#define JSVD_PATH_BLOCK_F64 1 /* block Jacobi (32-column blocks), FP64 MFMA Gram and rotate */
+#define JSVD_PATH_MIXED 2     /* FP32 MFMA block sweeps, FP64 re-orthogonalisation, FP64 block finishing */
+
+typedef struct jsvd_mixed_config
+{
+    int32_t path;                /* JSVD_PATH_* (default JSVD_PATH_MIXED) */
+    int32_t fp32_max_sweeps;     /* cap of the FP32 block sweeps (default 40) */
+    double fp32_tolerance;       /* FP32 phase threshold; <= 0 selects max(1e-6, 4 sqrt(m) 2^-24) */
+    int32_t orthogonalize_steps; /* Newton-Schulz steps applied to the FP32 V in FP64 (default 2) */
+    int32_t reserved;
+} jsvd_mixed_config;
+
+/* Per-solve report of a plan. */
+typedef struct jsvd_solve_info
+{
+    int32_t path;        /* path that ran */
+    int32_t sweeps_fp32; /* FP32 block sweeps (0 outside the mixed path) */
+    int32_t sweeps_fp64; /* FP64 sweeps (column-pair or block); also the value returned as `sweeps` */
+    int32_t converged;   /* 1 when the FP64 phase ended on a rotation-free sweep */
+    double offmax_fp32;  /* max pair cosine seen in the last FP32 sweep */
+    double offmax_fp64;  /* max pair cosine seen in the last FP64 sweep */
+} jsvd_solve_info;
+
+void jsvd_mixed_config_default(jsvd_mixed_config *mixed);
+
+/*
+ * Plan for the path selected by mixed->path (NULL: the plain FP64 path, identical to jsvd_plan_create).
+ * The block paths need n >= 64 (two 32-column blocks); smaller n is rejected with
+ * JSVD_ERR_INVALID_ARGUMENT.
+ */
+int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed, void *stream,
+                        jsvd_plan **plan);
+
+/* Report of the last solve on this plan. */
+int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info);
+
+/* One-shot host-pointer solve through the path selected by `mixed` (NULL: plain FP64). */
+int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
+                   double *u, double *sigma, double *v, jsvd_solve_info *info);
+
 /*
  * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
  * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "mixed.hpp"
 #include "plain_f64.hpp"
 
 namespace jsvd
@@ -128,9 +129,14 @@ struct jsvd_plan
     int last_sweeps = 0;
     std::vector<int> last_rot;
     std::vector<double> last_off;
+    // Block / mixed paths (JSVD_PATH_BLOCK_F64, JSVD_PATH_MIXED).
+    int path = JSVD_PATH_PLAIN_F64;
+    std::unique_ptr<jsvd::BlockSolver> block;
+    jsvd_solve_info last_info{};
 
     ~jsvd_plan()
     {
+        block.reset();
         (void)hipFree(a_col);
         (void)hipFree(v_col);
         (void)hipFree(rot_count);
@@ -151,6 +157,25 @@ struct jsvd_plan
         {
             (void)hipStreamDestroy(stream);
         }
+    }
+
+    void create_block(int64_t rows, int64_t cols, const jsvd_config &config, const jsvd_mixed_config &mixed,
+                      hipStream_t user_stream)
+    {
+        m = rows;
+        n = cols;
+        cfg = config;
+        path = mixed.path;
+        if (user_stream)
+        {
+            stream = user_stream;
+        }
+        else
+        {
+            JSVD_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        block = std::make_unique<jsvd::BlockSolver>(m, n, cfg, mixed, stream);
     }
 
     void create(int64_t rows, int64_t cols, const jsvd_config &config, hipStream_t user_stream)
@@ -186,6 +211,28 @@ struct jsvd_plan
 
     // Device-resident solve: transpose in, sweep loop, sigma/U extraction, V transpose out.
     void solve_device(const double *a_rm, double *u_rm, double *sigma, double *v_rm, int *sweeps_out)
+    {
+        if (block)
+        {
+            block->solve_device(a_rm, u_rm, sigma, v_rm, last_info);
+            last_sweeps = last_info.sweeps_fp64;
+            last_rot.clear();
+            last_off.clear();
+            if (sweeps_out)
+            {
+                *sweeps_out = last_info.sweeps_fp64;
+            }
+            return;
+        }
+        solve_plain(a_rm, u_rm, sigma, v_rm, sweeps_out);
+        last_info = jsvd_solve_info{};
+        last_info.path = JSVD_PATH_PLAIN_F64;
+        last_info.sweeps_fp64 = last_sweeps;
+        last_info.converged = (!last_rot.empty() && last_rot.back() == 0) ? 1 : 0;
+        last_info.offmax_fp64 = last_off.empty() ? 0.0 : last_off.back();
+    }
+
+    void solve_plain(const double *a_rm, double *u_rm, double *sigma, double *v_rm, int *sweeps_out)
     {
         const int max_sweeps = cfg.max_sweeps;
         const double eps = cfg.epsilon;
@@ -429,6 +476,103 @@ int jsvd_plain_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg
         if (sweeps)
         {
             *sweeps = s;
+        }
+    });
+}
+
+void jsvd_mixed_config_default(jsvd_mixed_config *mixed)
+{
+    if (!mixed)
+    {
+        return;
+    }
+    std::memset(mixed, 0, sizeof(*mixed));
+    mixed->path = JSVD_PATH_MIXED;
+    mixed->fp32_max_sweeps = 40;
+    mixed->fp32_tolerance = 0.0;
+    mixed->orthogonalize_steps = 2;
+}
+
+namespace
+{
+    void validate_mixed(const jsvd_mixed_config &mx, int64_t n)
+    {
+        if (mx.path != JSVD_PATH_PLAIN_F64 && n <= jsvd::kB)
+        {
+            jsvd::throw_invalid("block and mixed paths need at least two 32-column blocks (n >= 33).");
+        }
+        if (mx.path < JSVD_PATH_PLAIN_F64 || mx.path > JSVD_PATH_MIXED)
+        {
+            jsvd::throw_invalid("unknown solver path.");
+        }
+        if (mx.path == JSVD_PATH_MIXED && (mx.fp32_max_sweeps <= 0 || mx.orthogonalize_steps < 1 ||
+                                           mx.orthogonalize_steps > 8 || !(mx.fp32_tolerance < 1.0)))
+        {
+            jsvd::throw_invalid("invalid mixed-precision configuration.");
+        }
+    }
+
+    std::unique_ptr<jsvd_plan> make_plan(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
+                                         hipStream_t stream)
+    {
+        jsvd_config c;
+        if (cfg)
+        {
+            c = *cfg;
+        }
+        else
+        {
+            jsvd_config_default(&c);
+        }
+        jsvd::validate_shape(m, n);
+        jsvd::validate_config(c);
+        auto p = std::make_unique<jsvd_plan>();
+        if (mixed && mixed->path != JSVD_PATH_PLAIN_F64)
+        {
+            validate_mixed(*mixed, n);
+            p->create_block(m, n, c, *mixed, stream);
+        }
+        else
+        {
+            if (mixed)
+            {
+                validate_mixed(*mixed, n);
+            }
+            p->create(m, n, c, stream);
+        }
+        return p;
+    }
+} // namespace
+
+int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed, void *stream,
+                        jsvd_plan **plan)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan output pointer must not be null.");
+        *plan = nullptr;
+        *plan = make_plan(m, n, cfg, mixed, static_cast<hipStream_t>(stream)).release();
+    });
+}
+
+int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr && info != nullptr, "plan and info must not be null.");
+        *info = plan->last_info;
+    });
+}
+
+int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
+                   double *u, double *sigma, double *v, jsvd_solve_info *info)
+{
+    return jsvd::guarded([&] {
+        require(a && u && sigma && v, "host buffers must not be null.");
+        auto p = make_plan(m, n, cfg, mixed, nullptr);
+        int s = 0;
+        p->solve_host(a, u, sigma, v, &s);
+        if (info)
+        {
+            *info = p->last_info;
         }
     });
 }

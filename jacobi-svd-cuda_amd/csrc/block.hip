@@ -1,0 +1,519 @@
+// block.hip -- block one-sided Jacobi kernels (FP32 and FP64) for the mixed-precision path.
+//
+// Columns are grouped into blocks of kB = 32; a round pairs blocks (I, J) by the same round-robin circle
+// the plain path uses for columns.  For every block pair, one round is:
+//
+//   block_gram_kernel    G_s = X_s^T X_s for each row slab X_s (kH rows x kW = 64 columns of [A_I A_J]) on
+//                        MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64), accumulating in T
+//   block_eigen_kernel   G = sum_s G_s, then one cyclic Jacobi sweep over the 64 columns carried out in Gram
+//                        space (two-sided 2x2 updates of G), rotation parameters (tau, t, c, s) in FP64 with
+//                        the reference skip test; accumulates the 64 x 64 rotation Q
+//   block_apply_kernel   [A_I A_J] <- [A_I A_J] Q and [V_I V_J] <- [V_I V_J] Q per row slab on MFMA
+//
+// Layout: X (A or V) column-major with leading dimension ld (a multiple of kH, padding rows zero); the column
+// count is padded to a multiple of kB with zero columns (their Gram entries are zero, so they never rotate).
+#include "block.hpp"
+
+namespace jsvd
+{
+    namespace
+    {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+        // One 16x16x4 MFMA step.  Lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four accumulator values
+        // of lane l are D[row(l, v)][l%16] with row = 4*(l/16)+v for f32 and row = l/16 + 4*v for f64.
+        template <typename T>
+        struct Mfma;
+
+        template <>
+        struct Mfma<float>
+        {
+            using Acc = f32x4;
+            __device__ static Acc step(float a, float b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+            __device__ static int row(int lane, int v) { return 4 * (lane / 16) + v; }
+        };
+
+        template <>
+        struct Mfma<double>
+        {
+            using Acc = f64x4;
+            __device__ static Acc step(double a, double b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+            __device__ static int row(int lane, int v) { return lane / 16 + 4 * v; }
+        };
+
+        __device__ inline void block_pair(int slot, int round, int Nb, int &I, int &J)
+        {
+            I = dev::circle_at(slot, round, Nb);
+            J = dev::circle_at(Nb - 1 - slot, round, Nb);
+        }
+
+        // Loads rows [r0, r0+kH) of the kW columns [X_I X_J] into LDS as xs[c][r] (column-major, padded).
+        template <typename T>
+        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, int I, int J, int64_t r0,
+                                         T (*xs)[kH + 1])
+        {
+            // kW * kH elements, 256 threads: each thread moves kW*kH/256 elements; consecutive threads read
+            // consecutive rows of one column (coalesced).  All loads are issued before the LDS stores so
+            // they are in flight together.
+            constexpr int kPer = kW * kH / 256;
+            T tmp[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                const int e = threadIdx.x + 256 * k;
+                const int c = e / kH;
+                const int r = e % kH;
+                const int64_t col = (c < kB) ? (static_cast<int64_t>(I) * kB + c) : (static_cast<int64_t>(J) * kB + c - kB);
+                tmp[k] = x[col * ld + r0 + r];
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                const int e = threadIdx.x + 256 * k;
+                xs[e / kH][e % kH] = tmp[k];
+            }
+        }
+
+        template <typename T>
+        __global__ __launch_bounds__(256) void block_gram_kernel(const T *__restrict__ a, int64_t lda, int nb, int Nb,
+                                                                 int round, int slabs, T *__restrict__ partial,
+                                                                 const int *__restrict__ skip_if_zero)
+        {
+            __shared__ T xs[kW][kH + 1];
+            if (skip_if_zero != nullptr && *skip_if_zero == 0)
+            {
+                return;
+            }
+            const int slot = blockIdx.x;
+            const int s = blockIdx.y;
+            int I, J;
+            block_pair(slot, round, Nb, I, J);
+            if (I >= nb || J >= nb)
+            {
+                return;
+            }
+            load_slab(a, lda, I, J, static_cast<int64_t>(s) * kH, xs);
+            __syncthreads();
+            const int lane = threadIdx.x % 64;
+            const int wave = threadIdx.x / 64;
+            const int i0 = 16 * wave; // this wave computes G rows i0..i0+15, all kW columns
+            typename Mfma<T>::Acc acc[4];
+            for (int t = 0; t < 4; ++t)
+            {
+                acc[t] = typename Mfma<T>::Acc{0, 0, 0, 0};
+            }
+            for (int r0 = 0; r0 < kH; r0 += 4)
+            {
+                const int kr = r0 + lane / 16;
+                const T av = xs[i0 + lane % 16][kr];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+                    const T bv = xs[16 * t + lane % 16][kr];
+                    acc[t] = Mfma<T>::step(av, bv, acc[t]);
+                }
+            }
+            T *out = partial + (static_cast<int64_t>(slot) * slabs + s) * (kW * kW);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int i = i0 + Mfma<T>::row(lane, v);
+                    const int j = 16 * t + lane % 16;
+                    out[i * kW + j] = acc[t][v];
+                }
+            }
+        }
+
+        // Reduction of the slab partials and one inner cyclic Jacobi sweep in Gram space.
+        //
+        // G is symmetric and kept only in its upper triangle (g[min][max]).  Each inner round pairs the 64
+        // columns by the round-robin circle; thread t < 32 forms the rotation of pair t in FP64, then the
+        // 528 2x2 blocks (pair t rows, pair u columns, t <= u) get the two-sided update J_t^T G J_u and the
+        // 2048 (row, pair) entries of Q the one-sided update Q J_u.  Thread tid < 528 owns 2x2 block tid of
+        // the upper triangle for the whole sweep; the column indices of a round come from the closed form of
+        // the circle.  A block pair none of whose 2016 column pairs passes the skip test is left untouched.
+        constexpr int kEigenThreads = 1024; // one thread per 2x2 block (528) and per Q row pair (2 x 1024)
+        constexpr int kTriBlocks = (kW / 2) * (kW / 2 + 1) / 2;
+
+        template <typename T>
+        __global__ __launch_bounds__(kEigenThreads) void block_eigen_kernel(
+            const T *__restrict__ partial, int slabs, int nb, int Nb, int round, double eps, T *__restrict__ q_out,
+            int *__restrict__ apply_flag, int *__restrict__ rot_count, unsigned long long *__restrict__ offmax,
+            const int *__restrict__ skip_if_zero)
+        {
+            __shared__ T g[kW][kW + 1];
+            __shared__ T q[kW][kW + 1];
+            __shared__ double cs[kW / 2][2];
+            __shared__ int rotated[kW / 2];
+            __shared__ double red[kEigenThreads / 64];
+            __shared__ int any;
+            __shared__ int needed;
+            if (skip_if_zero != nullptr && *skip_if_zero == 0)
+            {
+                return;
+            }
+            const int slot = blockIdx.x;
+            int I, J;
+            block_pair(slot, round, Nb, I, J);
+            if (I >= nb || J >= nb)
+            {
+                return;
+            }
+            const int tid = threadIdx.x;
+            // This thread's 2x2 block (t, u), t <= u, of the upper triangle of the 32 x 32 block grid.
+            int bt = 0, bu = 0;
+            if (tid < kTriBlocks)
+            {
+                int rem = tid;
+                while (rem >= kW / 2 - bt)
+                {
+                    rem -= kW / 2 - bt;
+                    ++bt;
+                }
+                bu = bt + rem;
+            }
+            const T *base = partial + static_cast<int64_t>(slot) * slabs * (kW * kW);
+            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            {
+                const int i = e / kW;
+                const int j = e % kW;
+                if (i <= j)
+                {
+                    // eight independent loads in flight per step; the summation order is fixed (slab order)
+                    T acc = 0;
+                    int s = 0;
+                    for (; s + 8 <= slabs; s += 8)
+                    {
+                        T part[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            part[k] = base[static_cast<int64_t>(s + k) * (kW * kW) + e];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            acc += part[k];
+                        }
+                    }
+                    for (; s < slabs; ++s)
+                    {
+                        acc += base[static_cast<int64_t>(s) * (kW * kW) + e];
+                    }
+                    g[i][j] = acc;
+                }
+                q[i][j] = (i == j) ? T(1) : T(0);
+            }
+            if (tid == 0)
+            {
+                any = 0;
+                needed = 0;
+            }
+            __syncthreads();
+            // Off-orthogonality of the pair's 64 columns before the sweep (convergence measure), and whether
+            // any pair passes the skip test at all: if none does, no rotation can happen in the sweep (G never
+            // changes) and the sweep is skipped outright.
+            double best = 0.0;
+            bool need = false;
+            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            {
+                const int i = e / kW;
+                const int j = e % kW;
+                if (i < j)
+                {
+                    const double gii = static_cast<double>(g[i][i]);
+                    const double gjj = static_cast<double>(g[j][j]);
+                    const double gij = static_cast<double>(g[i][j]);
+                    best = fmax(best, dev::pair_cosine(gii, gjj, gij));
+                    const double rhs = eps * sqrt(fmax(gii, 0.0) * fmax(gjj, 0.0));
+                    need = need || (fabs(gij) > rhs && fabs(gij) > 1.0e-300);
+                }
+            }
+            best = dev::wave_max(best);
+            if ((tid % 64) == 0)
+            {
+                red[tid / 64] = best;
+            }
+            if (need)
+            {
+                needed = 1;
+            }
+            __syncthreads();
+            int my_rotations = 0;
+            for (int rr = 0; needed && rr < kW - 1; ++rr)
+            {
+                if (tid < kW / 2)
+                {
+                    const int p = dev::circle_at_fast(tid, rr, kW);
+                    const int qq = dev::circle_at_fast(kW - 1 - tid, rr, kW);
+                    double c, s;
+                    const bool rot = dev::rotation_params_block(static_cast<double>(g[p][p]),
+                                                                static_cast<double>(g[qq][qq]),
+                                                                static_cast<double>(g[min(p, qq)][max(p, qq)]), eps, c, s);
+                    cs[tid][0] = c;
+                    cs[tid][1] = s;
+                    rotated[tid] = rot ? 1 : 0;
+                    my_rotations += rot ? 1 : 0;
+                }
+                __syncthreads();
+                if (tid < kTriBlocks && (rotated[bt] || rotated[bu]))
+                {
+                    const int x1 = dev::circle_at_fast(bt, rr, kW), x2 = dev::circle_at_fast(kW - 1 - bt, rr, kW);
+                    const int y1 = dev::circle_at_fast(bu, rr, kW), y2 = dev::circle_at_fast(kW - 1 - bu, rr, kW);
+                    const T ct = static_cast<T>(cs[bt][0]), st = static_cast<T>(cs[bt][1]);
+                    const T cu = static_cast<T>(cs[bu][0]), su = static_cast<T>(cs[bu][1]);
+                    T &e11 = g[min(x1, y1)][max(x1, y1)];
+                    T &e12 = g[min(x1, y2)][max(x1, y2)];
+                    T &e21 = g[min(x2, y1)][max(x2, y1)];
+                    T &e22 = g[min(x2, y2)][max(x2, y2)];
+                    const T g11 = e11, g12 = e12, g21 = e21, g22 = e22;
+                    // rows by J_t^T (x' = c x - s y on row pairs), then columns by J_u
+                    const T h11 = ct * g11 - st * g21, h12 = ct * g12 - st * g22;
+                    const T h21 = st * g11 + ct * g21, h22 = st * g12 + ct * g22;
+                    const T n11 = cu * h11 - su * h12;
+                    const T n12 = su * h11 + cu * h12;
+                    const T n21 = cu * h21 - su * h22;
+                    const T n22 = su * h21 + cu * h22;
+                    e11 = n11;
+                    e12 = n12;
+                    e22 = n22;
+                    if (bt != bu)
+                    {
+                        e21 = n21; // on the diagonal block e21 aliases e12 (one symmetric entry)
+                    }
+                }
+                {
+                    // Q J_u: thread handles pair u = tid % 32 on rows k0 = tid / 32 and k0 + 32 (the loads of
+                    // both rows are issued before the stores).
+                    const int u = tid % (kW / 2);
+                    const int k0 = tid / (kW / 2);
+                    if (rotated[u])
+                    {
+                        const int y1 = dev::circle_at_fast(u, rr, kW), y2 = dev::circle_at_fast(kW - 1 - u, rr, kW);
+                        const T cu = static_cast<T>(cs[u][0]), su = static_cast<T>(cs[u][1]);
+                        const T x0 = q[k0][y1], z0 = q[k0][y2];
+                        const T x1 = q[k0 + kW / 2][y1], z1 = q[k0 + kW / 2][y2];
+                        q[k0][y1] = cu * x0 - su * z0;
+                        q[k0][y2] = su * x0 + cu * z0;
+                        q[k0 + kW / 2][y1] = cu * x1 - su * z1;
+                        q[k0 + kW / 2][y2] = su * x1 + cu * z1;
+                    }
+                }
+                __syncthreads();
+            }
+            if (tid < kW / 2 && my_rotations > 0)
+            {
+                atomicAdd(&any, my_rotations);
+            }
+            __syncthreads();
+            T *qo = q_out + static_cast<int64_t>(slot) * (kW * kW);
+            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            {
+                qo[e] = q[e / kW][e % kW]; // row-major Q[c][j]
+            }
+            if (tid == 0)
+            {
+                apply_flag[slot] = any;
+                if (any > 0 && rot_count)
+                {
+                    atomicAdd(rot_count, any);
+                }
+                if (offmax)
+                {
+                    double m = 0.0;
+                    for (int w = 0; w < kEigenThreads / 64; ++w)
+                    {
+                        m = fmax(m, red[w]);
+                    }
+                    dev::atomic_max_nonneg(offmax, m);
+                }
+            }
+        }
+
+        // Y = X Q for one row slab of [X_I X_J]; grid.y spans the slabs of A followed by those of V.
+        template <typename T>
+        __global__ __launch_bounds__(256) void block_apply_kernel(T *__restrict__ a, int64_t lda, int slabs_a,
+                                                                  T *__restrict__ v, int64_t ldv, int nb, int Nb,
+                                                                  int round, const T *__restrict__ q_in,
+                                                                  const int *__restrict__ apply_flag,
+                                                                  const int *__restrict__ skip_if_zero)
+        {
+            __shared__ T xs[kW][kH + 1];
+            __shared__ T qs[kW][kW + 1];
+            if (skip_if_zero != nullptr && *skip_if_zero == 0)
+            {
+                return;
+            }
+            const int slot = blockIdx.x;
+            int I, J;
+            block_pair(slot, round, Nb, I, J);
+            if (I >= nb || J >= nb || apply_flag[slot] == 0)
+            {
+                return;
+            }
+            const int sy = blockIdx.y;
+            T *x = (sy < slabs_a) ? a : v;
+            const int64_t ld = (sy < slabs_a) ? lda : ldv;
+            const int64_t r0 = static_cast<int64_t>((sy < slabs_a) ? sy : sy - slabs_a) * kH;
+            load_slab(x, ld, I, J, r0, xs);
+            const T *qg = q_in + static_cast<int64_t>(slot) * (kW * kW);
+            for (int e = threadIdx.x; e < kW * kW; e += 256)
+            {
+                qs[e / kW][e % kW] = qg[e];
+            }
+            __syncthreads();
+            const int lane = threadIdx.x % 64;
+            const int wave = threadIdx.x / 64;
+            // wave computes output rows [32*wave, 32*wave+32) (two 16-row tiles) x all 64 columns (four tiles)
+            typename Mfma<T>::Acc acc[2][4];
+            for (int a2 = 0; a2 < 2; ++a2)
+            {
+                for (int t = 0; t < 4; ++t)
+                {
+                    acc[a2][t] = typename Mfma<T>::Acc{0, 0, 0, 0};
+                }
+            }
+            for (int c0 = 0; c0 < kW; c0 += 4)
+            {
+                const int kc = c0 + lane / 16;
+                const T av0 = xs[kc][32 * wave + lane % 16];
+                const T av1 = xs[kc][32 * wave + 16 + lane % 16];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+                    const T bv = qs[kc][16 * t + lane % 16];
+                    acc[0][t] = Mfma<T>::step(av0, bv, acc[0][t]);
+                    acc[1][t] = Mfma<T>::step(av1, bv, acc[1][t]);
+                }
+            }
+            __syncthreads(); // every wave has read xs; overwrite it with the product
+#pragma unroll
+            for (int a2 = 0; a2 < 2; ++a2)
+            {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+#pragma unroll
+                    for (int vv = 0; vv < 4; ++vv)
+                    {
+                        const int r = 32 * wave + 16 * a2 + Mfma<T>::row(lane, vv);
+                        const int c = 16 * t + lane % 16;
+                        xs[c][r] = acc[a2][t][vv];
+                    }
+                }
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < kW * kH; e += 256)
+            {
+                const int c = e / kH;
+                const int r = e % kH;
+                const int64_t col = (c < kB) ? (static_cast<int64_t>(I) * kB + c) : (static_cast<int64_t>(J) * kB + c - kB);
+                x[col * ld + r0 + r] = xs[c][r];
+            }
+        }
+
+        template <typename Tin, typename Tout>
+        __global__ void convert_kernel(const Tin *__restrict__ src, Tout *__restrict__ dst, int64_t count)
+        {
+            const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i < count)
+            {
+                dst[i] = static_cast<Tout>(src[i]);
+            }
+        }
+
+        // Copies the leading rows x cols block of a column-major src (lds) into dst (ldd), zero-filling
+        // dst rows rows..ldd-1 and columns cols..dcols-1 except for a unit diagonal where `identity_pad`.
+        template <typename T>
+        __global__ void pad_copy_kernel(const T *__restrict__ src, int64_t lds, int64_t rows, int64_t cols,
+                                        T *__restrict__ dst, int64_t ldd, int64_t dcols, int identity_pad)
+        {
+            const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i >= ldd * dcols)
+            {
+                return;
+            }
+            const int64_t c = i / ldd;
+            const int64_t r = i % ldd;
+            T x = 0;
+            if (r < rows && c < cols)
+            {
+                x = src[c * lds + r];
+            }
+            else if (identity_pad && c >= cols && r == c)
+            {
+                x = 1;
+            }
+            dst[i] = x;
+        }
+
+        template <typename T>
+        __global__ void identity_kernel_t(T *__restrict__ v, int64_t ldv, int64_t cols)
+        {
+            const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i >= ldv * cols)
+            {
+                return;
+            }
+            v[i] = (i / ldv == i % ldv) ? T(1) : T(0);
+        }
+    } // namespace
+
+    template <typename T>
+    void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
+                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
+                            hipStream_t stream)
+    {
+        const int pairs = g.Nb / 2;
+        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, g.slabs_a), dim3(256), 0, stream, a, g.lda, g.nb, g.Nb,
+                           round, g.slabs_a, partial, skip_if_zero);
+        JSVD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream, partial, g.slabs_a, g.nb, g.Nb,
+                           round, eps, q, apply_flag, rot_count, offmax, skip_if_zero);
+        JSVD_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, g.slabs_a + g.slabs_v), dim3(256), 0, stream, a, g.lda,
+                           g.slabs_a, v, g.ldv, g.nb, g.Nb, round, static_cast<const T *>(q),
+                           static_cast<const int *>(apply_flag), skip_if_zero);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    template void launch_block_round<float>(const BlockGeometry &, float *, float *, int, double, float *, float *,
+                                            int *, int *, unsigned long long *, const int *, hipStream_t);
+    template void launch_block_round<double>(const BlockGeometry &, double *, double *, int, double, double *,
+                                             double *, int *, int *, unsigned long long *, const int *, hipStream_t);
+
+    void launch_convert_f64_to_f32(const double *src, float *dst, int64_t count, hipStream_t stream)
+    {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256),
+                           0, stream, src, dst, count);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    void launch_convert_f32_to_f64(const float *src, double *dst, int64_t count, hipStream_t stream)
+    {
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256),
+                           0, stream, src, dst, count);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    void launch_pad_copy_f64(const double *src, int64_t lds, int64_t rows, int64_t cols, double *dst, int64_t ldd,
+                             int64_t dcols, bool identity_pad, hipStream_t stream)
+    {
+        const int64_t total = ldd * dcols;
+        hipLaunchKernelGGL(pad_copy_kernel<double>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                           stream, src, lds, rows, cols, dst, ldd, dcols, identity_pad ? 1 : 0);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    void launch_identity_f32(float *v, int64_t ldv, int64_t cols, hipStream_t stream)
+    {
+        const int64_t total = ldv * cols;
+        hipLaunchKernelGGL(identity_kernel_t<float>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                           stream, v, ldv, cols);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+} // namespace jsvd

@@ -1,0 +1,56 @@
+// block.hpp -- block one-sided Jacobi (mixed-precision path): geometry and launchers (block.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace jsvd
+{
+    constexpr int kB = 32;      // block width (columns)
+    constexpr int kW = 2 * kB;  // columns of a block pair
+    constexpr int kH = 128;     // row-slab height of the Gram and apply tiles
+
+    // Padded geometry of one matrix for the block kernels: lda / ldv are multiples of kH, the column count
+    // is padded to nb * kB with zero columns.
+    struct BlockGeometry
+    {
+        int64_t m = 0;      // rows of A
+        int64_t n = 0;      // columns of A
+        int64_t n_pad = 0;  // nb * kB
+        int nb = 0;         // blocks
+        int Nb = 0;         // nb rounded up to even (round-robin circle size)
+        int64_t lda = 0;    // rows of the padded A (multiple of kH)
+        int64_t ldv = 0;    // rows of the padded V (multiple of kH, >= n_pad)
+        int slabs_a = 0;    // lda / kH
+        int slabs_v = 0;    // ldv / kH
+
+        static BlockGeometry make(int64_t m, int64_t n)
+        {
+            BlockGeometry g;
+            g.m = m;
+            g.n = n;
+            g.nb = static_cast<int>((n + kB - 1) / kB);
+            g.n_pad = static_cast<int64_t>(g.nb) * kB;
+            g.Nb = g.nb + (g.nb & 1);
+            g.lda = round_up(m, kH);
+            g.ldv = round_up(g.n_pad, kH);
+            g.slabs_a = static_cast<int>(g.lda / kH);
+            g.slabs_v = static_cast<int>(g.ldv / kH);
+            return g;
+        }
+        [[nodiscard]] int rounds() const { return Nb - 1; }
+        [[nodiscard]] int pairs() const { return Nb / 2; }
+    };
+
+    // One block round (Gram partials, reduction + inner sweep, apply to A and V).  partial holds
+    // pairs * slabs_a * kW * kW elements, q pairs * kW * kW, apply_flag pairs ints.
+    template <typename T>
+    void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
+                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
+                            hipStream_t stream);
+
+    void launch_convert_f64_to_f32(const double *src, float *dst, int64_t count, hipStream_t stream);
+    void launch_convert_f32_to_f64(const float *src, double *dst, int64_t count, hipStream_t stream);
+    void launch_pad_copy_f64(const double *src, int64_t lds, int64_t rows, int64_t cols, double *dst, int64_t ldd,
+                             int64_t dcols, bool identity_pad, hipStream_t stream);
+    void launch_identity_f32(float *v, int64_t ldv, int64_t cols, hipStream_t stream);
+} // namespace jsvd

@@ -130,6 +130,42 @@ namespace jsvd::dev
         return false;
     }
 
+    // circle_at for 0 <= r < N-1 without an integer modulo (inner loops).
+    __device__ inline int circle_at_fast(int slot, int r, int N)
+    {
+        int k = slot - 1 - r;
+        k += (k < 0) ? (N - 1) : 0;
+        return (slot == 0) ? 0 : 1 + k;
+    }
+
+    // Same skip test as rotation_params, shorter dependent chain for the block paths' inner sweeps:
+    // with d = beta - alpha and g = 2 gamma scaled by 1 / max(|d|, |g|) (no overflow for any finite input),
+    // t = sign(d) g / (|d| + hypot(d, g)) and c = (|d| + hypot) / hypot(|d| + hypot, g), the same rotation as
+    // tau = d / g, t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2).
+    __device__ inline bool rotation_params_block(double app, double aqq, double apq, double eps, double &c,
+                                                 double &s)
+    {
+        const double rhs = eps * sqrt(fmax(app, 0.0) * fmax(aqq, 0.0));
+        if (fabs(apq) > rhs && fabs(apq) > 1.0e-300)
+        {
+            const double d0 = aqq - app;
+            const double g0 = 2.0 * apq;
+            const double inv = 1.0 / fmax(fabs(d0), fabs(g0));
+            const double d = d0 * inv;
+            const double g = g0 * inv;
+            const double hyp = sqrt(d * d + g * g);
+            const double den = fabs(d) + hyp;
+            const double t = ((d >= 0.0) ? g : -g) / den;
+            const double cosine = den / sqrt(den * den + g * g);
+            c = cosine;
+            s = t * cosine;
+            return true;
+        }
+        c = 1.0;
+        s = 0.0;
+        return false;
+    }
+
     // Off-orthogonality of a pair, |gamma| / sqrt(alpha beta), 0 when a column is zero.
     __host__ __device__ inline double pair_cosine(double app, double aqq, double apq)
     {

@@ -1,0 +1,52 @@
+// mixed.hpp -- block and mixed-precision one-sided Jacobi solvers (mixed.hip).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "block.hpp"
+
+namespace jsvd
+{
+    // Device-resident block solver for one shape.  Path JSVD_PATH_BLOCK_F64 runs FP64 block sweeps only;
+    // JSVD_PATH_MIXED runs FP32 block sweeps (MFMA Gram and rotate, FP32 accumulation) to the FP32 tolerance,
+    // re-orthogonalises the FP32 V in FP64 (Newton-Schulz), forms A V in FP64 and finishes with FP64 block
+    // sweeps to the caller's epsilon.
+    class BlockSolver
+    {
+    public:
+        BlockSolver(int64_t m, int64_t n, const jsvd_config &cfg, const jsvd_mixed_config &mixed, hipStream_t stream);
+        ~BlockSolver();
+        BlockSolver(const BlockSolver &) = delete;
+        BlockSolver &operator=(const BlockSolver &) = delete;
+
+        // d_a row-major m x n (not modified); outputs row-major U (m x n), sigma (n), V (n x n).
+        void solve_device(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
+
+    private:
+        template <typename T>
+        int run_sweeps(T *a, T *v, double eps, int max_sweeps, int *rot, unsigned long long *off, double &last_off);
+        void orthogonalize_v(double *v, double *w);
+
+        BlockGeometry g_;
+        jsvd_config cfg_;
+        jsvd_mixed_config mixed_;
+        hipStream_t stream_;
+        void *blas_ = nullptr; // rocblas_handle
+        double *a64_ = nullptr;
+        double *a64b_ = nullptr;
+        double *v64_ = nullptr;
+        double *w64_ = nullptr;
+        double *gram64_ = nullptr;
+        float *a32_ = nullptr;
+        float *v32_ = nullptr;
+        double *partial_ = nullptr; // reused as float for the FP32 phase
+        double *q_ = nullptr;
+        int *flags_ = nullptr;
+        int *rot_ = nullptr;                 // [2 * cap] rotation counts (FP32 sweeps, then FP64 sweeps)
+        unsigned long long *off_ = nullptr;  // [2 * cap]
+        int *h_rot_ = nullptr;               // pinned mirror
+        int cap_ = 0;
+        hipEvent_t done_[2] = {nullptr, nullptr};
+    };
+} // namespace jsvd

@@ -53,6 +53,50 @@ class JsvdConfig(ctypes.Structure):
     ]
 
 
+PATH_PLAIN_F64, PATH_BLOCK_F64, PATH_MIXED = 0, 1, 2
+PATH_NAMES = {PATH_PLAIN_F64: "plain_f64", PATH_BLOCK_F64: "block_f64", PATH_MIXED: "mixed_f32_f64"}
+
+
+class MixedConfig(ctypes.Structure):
+    """jsvd_mixed_config (include/jsvd.h): solver path and FP32-phase knobs."""
+    _fields_ = [
+        ("path", ctypes.c_int32),
+        ("fp32_max_sweeps", ctypes.c_int32),
+        ("fp32_tolerance", ctypes.c_double),
+        ("orthogonalize_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class SolveInfo(ctypes.Structure):
+    """jsvd_solve_info (include/jsvd.h): per-solve report."""
+    _fields_ = [
+        ("path", ctypes.c_int32),
+        ("sweeps_fp32", ctypes.c_int32),
+        ("sweeps_fp64", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("offmax_fp32", ctypes.c_double),
+        ("offmax_fp64", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
+                 fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None) -> MixedConfig:
+    mx = MixedConfig()
+    lib().jsvd_mixed_config_default(ctypes.byref(mx))
+    mx.path = path
+    if fp32_max_sweeps is not None:
+        mx.fp32_max_sweeps = fp32_max_sweeps
+    if fp32_tolerance is not None:
+        mx.fp32_tolerance = fp32_tolerance
+    if orthogonalize_steps is not None:
+        mx.orthogonalize_steps = orthogonalize_steps
+    return mx
+
+
 @dataclasses.dataclass
 class JacobiSvdConfig:
     """Python mirror of jacobi::svd::JacobiSvdConfig with the reference defaults (kernels.hpp:83-123)."""
@@ -130,10 +174,17 @@ def lib() -> ctypes.CDLL:
                                      _vp, _vp]
         L.jsvd_rotation_params_f64.argtypes = [_vp, _vp, _vp, _i64, ctypes.c_double, _vp, _vp, _vp, _vp]
         L.jsvd_round_robin_pairs.argtypes = [_i64, ctypes.c_int32, _i32p]
+        L.jsvd_mixed_config_default.argtypes = [ctypes.POINTER(MixedConfig)]
+        L.jsvd_mixed_config_default.restype = None
+        L.jsvd_plan_create_ex.argtypes = [_i64, _i64, ctypes.POINTER(JsvdConfig), ctypes.POINTER(MixedConfig), _vp,
+                                          ctypes.POINTER(_vp)]
+        L.jsvd_plan_last_info.argtypes = [_vp, ctypes.POINTER(SolveInfo)]
+        L.jsvd_solve_f64.argtypes = [_vp, _i64, _i64, ctypes.POINTER(JsvdConfig), ctypes.POINTER(MixedConfig), _vp,
+                                     _vp, _vp, ctypes.POINTER(SolveInfo)]
         L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
         for name in declared_symbols():
-            if name not in ("jsvd_config_default", "jsvd_last_error", "jsvd_version"):
+            if name not in ("jsvd_config_default", "jsvd_last_error", "jsvd_version", "jsvd_mixed_config_default"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -193,6 +244,22 @@ def round_robin_pairs(n: int, rnd: int) -> list[tuple[int, int]]:
     return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(k)]
 
 
+def solve(a, config: Optional[JacobiSvdConfig] = None, path: int = PATH_MIXED,
+          mixed: Optional[MixedConfig] = None):
+    """One-shot host-array solve through the selected path (jsvd_solve_f64).  Returns (result, SolveInfo)."""
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    m, n = arr.shape
+    u = np.zeros((m, n))
+    s = np.zeros(n)
+    v = np.zeros((n, n))
+    info = SolveInfo()
+    cfg = _as_c_config(config)
+    mx = mixed if mixed is not None else mixed_config(path)
+    check(lib().jsvd_solve_f64(arr.ctypes.data, m, n, ctypes.byref(cfg), ctypes.byref(mx), u.ctypes.data,
+                               s.ctypes.data, v.ctypes.data, ctypes.byref(info)))
+    return JacobiSvdResult(m, n, u, s, v, info.sweeps_fp64), info
+
+
 def _stream_handle(stream=None) -> int:
     import torch
     if stream is None:
@@ -207,13 +274,21 @@ def _ptr(t) -> int:
 class Plan:
     """Reusable solver workspace for one shape (jsvd_plan_*)."""
 
-    def __init__(self, m: int, n: int, config=None, stream=None):
+    def __init__(self, m: int, n: int, config=None, stream=None, path: int = PATH_PLAIN_F64,
+                 mixed: Optional[MixedConfig] = None):
         self.m, self.n = int(m), int(n)
         self._cfg = _as_c_config(config)
+        self._mixed = mixed if mixed is not None else mixed_config(path)
         handle = _vp()
         s = _stream_handle(stream) if stream is not None else None
-        check(lib().jsvd_plan_create(self.m, self.n, ctypes.byref(self._cfg), s, ctypes.byref(handle)))
+        check(lib().jsvd_plan_create_ex(self.m, self.n, ctypes.byref(self._cfg), ctypes.byref(self._mixed), s,
+                                        ctypes.byref(handle)))
         self._h = handle
+
+    def last_info(self) -> SolveInfo:
+        info = SolveInfo()
+        check(lib().jsvd_plan_last_info(self._h, ctypes.byref(info)))
+        return info
 
     def close(self) -> None:
         if self._h:

@@ -1,0 +1,111 @@
+"""Block FP64 and mixed FP32/FP64 paths on cuda:0 against the CPU oracle and LAPACK dgesvj.
+
+These paths use a different (blocked) rotation order than the reference, so columns come out in a
+different Jacobi order: sigma is compared sorted, U and V through invariants and matched columns.
+Tolerances:
+  * sorted sigma vs the oracle / dgesvj: normwise max|d sigma| / sigma_max <= 1e-12 (the metric's
+    "sigma relerr <= 1e-12"), and per-sigma relative <= 1e-12 where sigma_i >= 1e-3 sigma_max;
+  * rho_A <= 1e-13, ||U^T U - I||_F and ||V^T V - I||_F <= 1e-11 (n up to 256);
+  * column-graded inputs (relative accuracy of Jacobi): per-sigma relative error vs dgesvj <= 1e-10.
+"""
+import numpy as np
+import pytest
+
+import jsvd
+import oracle
+from helpers import gaussian, match_columns, rho_a, rho_orth, sorted_desc
+
+pytestmark = pytest.mark.gpu
+
+PATHS = [jsvd.PATH_BLOCK_F64, jsvd.PATH_MIXED]
+
+
+def _solve(a, path, eps=1e-14, max_sweeps=60):
+    return jsvd.solve(a, jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=max_sweeps), path=path)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_gaussian_512x256_against_oracle(torch_cuda, path):
+    a = gaussian(512, 256, 0)
+    got, info = _solve(a, path, eps=1e-15, max_sweeps=40)
+    ref = oracle.svd(a, eps=1e-15, max_sweeps=30, threads=4)
+    s_ref = sorted_desc(ref.sigma)
+    s = sorted_desc(got.sigma)
+    assert info.converged == 1 and info.path == path
+    if path == jsvd.PATH_MIXED:
+        assert info.sweeps_fp32 > 0
+    assert np.max(np.abs(s - s_ref)) / s_ref[0] < 1e-12
+    assert np.max(np.abs(s - s_ref) / s_ref) < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-13
+    assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
+    # singular vectors equal the oracle's up to sign after sorting (well separated sigma)
+    o1 = np.argsort(-got.sigma, kind="stable")
+    o2 = np.argsort(-ref.sigma, kind="stable")
+    assert match_columns(got.v[:, o1], ref.v[:, o2]) < 1e-8
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_geometric_spectrum_1024x256(torch_cuda, path):
+    """A = Q1 diag(1 .. 1e-8) Q2^T: normwise sigma error vs the exact construction and vs dgesvj."""
+    from jsvd import workloads
+    a, s_true = workloads.tall_geometric(1024, 256)
+    got, info = _solve(a, path)
+    s = sorted_desc(got.sigma)
+    _, s_lapack, _, _ = oracle.dgesvj(a)
+    assert info.converged == 1
+    assert np.max(np.abs(s - s_true)) / s_true[0] < 1e-12
+    assert np.max(np.abs(s - s_lapack)) / s_lapack[0] < 1e-12
+    big = s_true >= 1e-3
+    assert np.max(np.abs(s - s_true)[big] / s_true[big]) < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-12
+    assert rho_orth(got.v) < 1e-11
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_column_graded_relative_accuracy(torch_cuda, path):
+    rng = np.random.default_rng(5)
+    m, n = 600, 160
+    a = rng.standard_normal((m, n)) * 10.0 ** (-12.0 * np.arange(n) / (n - 1))
+    got, info = _solve(a, path)
+    _, s_lapack, _, _ = oracle.dgesvj(a)
+    s = sorted_desc(got.sigma)
+    assert info.converged == 1
+    assert np.max(np.abs(s - s_lapack) / s_lapack) < 1e-10
+
+
+@pytest.mark.parametrize("m,n", [(70, 33), (129, 65), (300, 100), (257, 257)])
+@pytest.mark.parametrize("path", PATHS)
+def test_padded_shapes(torch_cuda, path, m, n):
+    """Column counts that are not multiples of the 32-column block and rows not multiples of the slab."""
+    a = gaussian(m, n, m * 7 + n)
+    got, info = _solve(a, path)
+    _, s_lapack, _, _ = oracle.dgesvj(a)
+    s = sorted_desc(got.sigma)
+    assert info.converged == 1
+    assert np.max(np.abs(s - s_lapack)) / s_lapack[0] < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-13
+    assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
+
+
+def test_mixed_plan_reuse_deterministic(torch_cuda):
+    torch = torch_cuda
+    m, n = 400, 96
+    a = gaussian(m, n, 3)
+    plan = jsvd.Plan(m, n, jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60), path=jsvd.PATH_MIXED)
+    d_a = torch.from_numpy(a).to("cuda")
+    outs = []
+    for _ in range(2):
+        u = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+        s = torch.zeros(n, dtype=torch.float64, device="cuda")
+        v = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        plan.solve_device(d_a, u, s, v)
+        outs.append((u.cpu().numpy(), s.cpu().numpy(), v.cpu().numpy(), plan.last_info().as_dict()))
+    for x, y in zip(outs[0][:3], outs[1][:3]):
+        assert np.array_equal(x, y)
+    assert outs[0][3] == outs[1][3]
+    plan.close()
+
+
+def test_block_paths_reject_small_n():
+    with pytest.raises(ValueError, match="two 32-column blocks"):
+        jsvd.solve(gaussian(40, 32, 0), path=jsvd.PATH_MIXED)
