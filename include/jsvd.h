@@ -150,6 +150,68 @@ int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info);
 int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
                    double *u, double *sigma, double *v, jsvd_solve_info *info);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Batched path: many independent matrices of different shapes solved by one persistent launch.  No
+ * reference counterpart as a batch (the reference solves one matrix per call, pipeline.cpp:711-739); per
+ * matrix it runs the reference algorithm (pair ordering src/kernels.cu:637-678, rotation and skip test
+ * src/kernels.cu:485-521, convergence rule src/kernels.cu:889-981, sigma/U extraction
+ * src/kernels.cu:586-629) in the requested precision, rotation parameters always in FP64.
+ * ------------------------------------------------------------------------------------------------- */
+#define JSVD_PRECISION_F64 0
+#define JSVD_PRECISION_F32 1
+
+/* One matrix of a batch.  Offsets are element offsets into the batch's packed FP64 buffers: A row-major
+ * rows x columns at a_offset, U row-major rows x columns at u_offset, sigma (columns) at sigma_offset,
+ * V row-major columns x columns at v_offset.  tolerance is the matrix's epsilon. */
+typedef struct jsvd_batch_item
+{
+    int64_t a_offset;
+    int64_t u_offset;
+    int64_t sigma_offset;
+    int64_t v_offset;
+    int32_t rows;
+    int32_t columns;
+    int32_t precision;  /* JSVD_PRECISION_* */
+    int32_t max_sweeps;
+    double tolerance;
+} jsvd_batch_item;
+
+/* Sizes (in doubles) of the packed buffers, checked by the kernel for every item. */
+typedef struct jsvd_batch_buffers
+{
+    const double *a;
+    double *u;
+    double *sigma;
+    double *v;
+    int32_t *sweeps; /* one per item; -1 for an item rejected by the kernel's bounds checks */
+    int64_t a_len;
+    int64_t u_len;
+    int64_t sigma_len;
+    int64_t v_len;
+} jsvd_batch_buffers;
+
+typedef struct jsvd_batch jsvd_batch;
+
+/* Batch solver for matrices up to max_rows x max_columns (max_columns <= max_rows <= 4096). */
+int jsvd_batch_create(int64_t max_rows, int64_t max_columns, void *stream, jsvd_batch **batch);
+int jsvd_batch_destroy(jsvd_batch *batch);
+
+/*
+ * Solves `count` items whose descriptors are in device memory (d_items), with device buffers `buf`.
+ * One persistent launch: each workgroup takes the next item from a device queue until none is left, so
+ * items are processed in descriptor order (put expensive ones first for load balance).  Each item's result
+ * is bit-identical whatever else is in the batch.  Returns after completion.
+ */
+int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
+                            const jsvd_batch_buffers *buf);
+
+/*
+ * Enqueues the same work on the batch's stream and returns at once; `done` (optional, a hipEvent_t) is
+ * recorded after it.  For the request loop, which overlaps several batches.
+ */
+int jsvd_batch_solve_async(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
+                           const jsvd_batch_buffers *buf, void *done);
+
 /*
  * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
  * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the

@@ -370,3 +370,124 @@ def offnorm(a_col, lda: int, m: int, n: int, out, stream=None) -> None:
 
 def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) -> None:
     check(lib().jsvd_u_sigma_f64(_ptr(a_col), lda, m, n, eps, _ptr(u), _ptr(sigma), _stream_handle(stream)))
+
+
+# ------------------------------------------------------------------------------------------------
+# batched path (jsvd_batch_*)
+# ------------------------------------------------------------------------------------------------
+PRECISION_F64, PRECISION_F32 = 0, 1
+
+BATCH_ITEM_DTYPE = np.dtype([
+    ("a_offset", "<i8"), ("u_offset", "<i8"), ("sigma_offset", "<i8"), ("v_offset", "<i8"),
+    ("rows", "<i4"), ("columns", "<i4"), ("precision", "<i4"), ("max_sweeps", "<i4"), ("tolerance", "<f8"),
+])
+assert BATCH_ITEM_DTYPE.itemsize == 56
+
+
+class BatchBuffers(ctypes.Structure):
+    """jsvd_batch_buffers (include/jsvd.h)."""
+    _fields_ = [
+        ("a", ctypes.c_void_p), ("u", ctypes.c_void_p), ("sigma", ctypes.c_void_p), ("v", ctypes.c_void_p),
+        ("sweeps", ctypes.c_void_p),
+        ("a_len", ctypes.c_int64), ("u_len", ctypes.c_int64), ("sigma_len", ctypes.c_int64), ("v_len", ctypes.c_int64),
+    ]
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    L.jsvd_batch_create.argtypes = [_i64, _i64, _vp, ctypes.POINTER(_vp)]
+    L.jsvd_batch_destroy.argtypes = [_vp]
+    L.jsvd_batch_solve_device.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers)]
+    L.jsvd_batch_solve_async.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers), _vp]
+    L._batch_bound = True
+
+
+class BatchPacked:
+    """Host packing of a list of matrices into the batch layout (offsets in doubles)."""
+
+    def __init__(self, mats, precisions, tolerances, max_sweeps):
+        k = len(mats)
+        self.items = np.zeros(k, dtype=BATCH_ITEM_DTYPE)
+        a_off = u_off = s_off = v_off = 0
+        for i, a in enumerate(mats):
+            m, n = a.shape
+            it = self.items[i]
+            it["a_offset"], it["u_offset"], it["sigma_offset"], it["v_offset"] = a_off, u_off, s_off, v_off
+            it["rows"], it["columns"] = m, n
+            it["precision"] = precisions[i]
+            it["max_sweeps"] = max_sweeps if np.isscalar(max_sweeps) else max_sweeps[i]
+            it["tolerance"] = tolerances[i]
+            a_off += m * n
+            u_off += m * n
+            s_off += n
+            v_off += n * n
+        self.a = np.zeros(a_off)
+        for i, a in enumerate(mats):
+            o = int(self.items[i]["a_offset"])
+            self.a[o:o + a.size] = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        self.lens = (a_off, u_off, s_off, v_off)
+
+    def cost_order(self):
+        """Queue order: most expensive first (m n^2), stable."""
+        cost = self.items["rows"].astype(np.float64) * self.items["columns"].astype(np.float64) ** 2
+        return np.argsort(-cost, kind="stable")
+
+
+class Batch:
+    """Persistent batched solver (jsvd_batch_*) for matrices up to max_rows x max_cols."""
+
+    def __init__(self, max_rows: int, max_cols: int, stream=None):
+        L = lib()
+        _bind_batch(L)
+        h = _vp()
+        s = _stream_handle(stream) if stream is not None else None
+        check(L.jsvd_batch_create(int(max_rows), int(max_cols), s, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            check(lib().jsvd_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_device(self, d_items, count, d_a, d_u, d_s, d_v, d_sweeps):
+        """All arguments are torch device tensors (items as uint8/int64 storage of BATCH_ITEM_DTYPE)."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        buf = BatchBuffers(d_a.data_ptr(), d_u.data_ptr(), d_s.data_ptr(), d_v.data_ptr(), d_sweeps.data_ptr(),
+                           d_a.numel(), d_u.numel(), d_s.numel(), d_v.numel())
+        check(lib().jsvd_batch_solve_device(self._h, d_items.data_ptr(), int(count), ctypes.byref(buf)))
+
+    def solve(self, mats, precisions, tolerances, max_sweeps=60, order=None):
+        """Host convenience: pack, upload, solve in one launch, download.  Returns (u list, s list, v list,
+        sweeps array) in input order."""
+        import torch
+        pk = BatchPacked(mats, precisions, tolerances, max_sweeps)
+        if order is None:
+            order = pk.cost_order()
+        items = pk.items[order]
+        d_items = torch.from_numpy(items.view(np.uint8).copy()).to("cuda")
+        d_a = torch.from_numpy(pk.a).to("cuda")
+        d_u = torch.zeros(max(pk.lens[1], 1), dtype=torch.float64, device="cuda")
+        d_s = torch.zeros(max(pk.lens[2], 1), dtype=torch.float64, device="cuda")
+        d_v = torch.zeros(max(pk.lens[3], 1), dtype=torch.float64, device="cuda")
+        d_sw = torch.zeros(len(mats), dtype=torch.int32, device="cuda")
+        self.solve_device(d_items, len(mats), d_a, d_u, d_s, d_v, d_sw)
+        u, s, v = d_u.cpu().numpy(), d_s.cpu().numpy(), d_v.cpu().numpy()
+        sw_q = d_sw.cpu().numpy()
+        sweeps = np.zeros(len(mats), dtype=np.int32)
+        sweeps[order] = sw_q
+        us, ss, vs = [], [], []
+        for it in pk.items:
+            m, n = int(it["rows"]), int(it["columns"])
+            uo, so, vo = int(it["u_offset"]), int(it["sigma_offset"]), int(it["v_offset"])
+            us.append(u[uo:uo + m * n].reshape(m, n))
+            ss.append(s[so:so + n].copy())
+            vs.append(v[vo:vo + n * n].reshape(n, n))
+        return us, ss, vs, sweeps
