@@ -212,6 +212,65 @@ int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, i
 int jsvd_batch_solve_async(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
                            const jsvd_batch_buffers *buf, void *done);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Request loop (continuous batching).  No reference counterpart (the reference pipeline solves a file of
+ * cases with one thread per case, pipeline.cpp:754-863).  Requests are admitted against an HBM budget;
+ * `lanes` worker lanes (a host thread and a HIP stream each) take jobs: one large request through a solver
+ * plan, or every admitted small request at once through one persistent batched launch.  Requests that arrive
+ * while jobs run are admitted as budget frees up and form the next job, so work of different requests
+ * overlaps on the device.  A request's result does not depend on what it was batched with.
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct jsvd_server_config
+{
+    int32_t lanes;             /* concurrent jobs (default 4) */
+    int32_t small_max_rows;    /* requests with rows <= this and columns <= small_max_columns are batched (256) */
+    int32_t small_max_columns; /* (128) */
+    int32_t max_batch;         /* requests per batched launch (4096) */
+    int64_t hbm_budget_bytes;  /* admission budget for request state on the device (64 GiB) */
+    int32_t max_sweeps;        /* per request (60) */
+    int32_t large_path;        /* JSVD_PATH_* for requests above the batch limits (JSVD_PATH_PLAIN_F64) */
+} jsvd_server_config;
+
+typedef struct jsvd_request_info
+{
+    int32_t status;     /* JSVD_OK, or the status of this request's failure */
+    int32_t sweeps;     /* executed sweeps (FP64 sweeps on the plan paths) */
+    int32_t precision;  /* JSVD_PRECISION_* chosen from the requested relative error */
+    int32_t batch_size; /* requests in the launch that served this one (1 for a plan) */
+    double queued_s;    /* submission -> start of its job */
+    double latency_s;   /* submission -> results ready on the host */
+} jsvd_request_info;
+
+typedef struct jsvd_server_stats
+{
+    int64_t submitted;
+    int64_t completed;
+    int64_t batches;        /* batched launches */
+    int64_t plans;          /* plan solves */
+    int64_t budget_in_use;  /* bytes currently reserved */
+    int64_t budget_peak;    /* highest reservation seen */
+    int64_t max_in_flight;  /* most requests admitted at once */
+} jsvd_server_stats;
+
+typedef struct jsvd_server jsvd_server;
+
+void jsvd_server_config_default(jsvd_server_config *cfg);
+int jsvd_server_create(const jsvd_server_config *cfg, jsvd_server **server);
+/* Waits for every submitted request, then releases the lanes. */
+int jsvd_server_destroy(jsvd_server *server);
+
+/*
+ * Submits a row-major m x n host matrix (copied before the call returns) with a requested relative error
+ * of sigma: relerr >= 1e-7 is served in FP32 (tolerance 1e-6), smaller in FP64 (tolerance
+ * max(1e-14, 2 sqrt(m) 2^-53)).  Returns at once with the request id.
+ */
+int jsvd_server_submit(jsvd_server *server, const double *a, int64_t m, int64_t n, double relerr, uint64_t *id);
+
+/* Blocks until request `id` is done, copies U (m x n), sigma (n), V (n x n) out and forgets the request. */
+int jsvd_server_wait(jsvd_server *server, uint64_t id, double *u, double *sigma, double *v, jsvd_request_info *info);
+
+int jsvd_server_get_stats(jsvd_server *server, jsvd_server_stats *stats);
+
 /*
  * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
  * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the

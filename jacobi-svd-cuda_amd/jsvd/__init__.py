@@ -491,3 +491,95 @@ class Batch:
             ss.append(s[so:so + n].copy())
             vs.append(v[vo:vo + n * n].reshape(n, n))
         return us, ss, vs, sweeps
+
+
+# ------------------------------------------------------------------------------------------------
+# request loop (jsvd_server_*)
+# ------------------------------------------------------------------------------------------------
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("lanes", ctypes.c_int32), ("small_max_rows", ctypes.c_int32), ("small_max_columns", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64), ("max_sweeps", ctypes.c_int32),
+        ("large_path", ctypes.c_int32),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("sweeps", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("batch_size", ctypes.c_int32), ("queued_s", ctypes.c_double), ("latency_s", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in ("submitted", "completed", "batches", "plans", "budget_in_use",
+                                              "budget_peak", "max_in_flight")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _bind_server(L):
+    if getattr(L, "_server_bound", False):
+        return
+    L.jsvd_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.jsvd_server_config_default.restype = None
+    L.jsvd_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+    L.jsvd_server_destroy.argtypes = [_vp]
+    L.jsvd_server_submit.argtypes = [_vp, _vp, _i64, _i64, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]
+    L.jsvd_server_wait.argtypes = [_vp, ctypes.c_uint64, _vp, _vp, _vp, ctypes.POINTER(RequestInfo)]
+    L.jsvd_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+    L._server_bound = True
+
+
+class Server:
+    """Continuous-batching request loop (jsvd_server_*).  submit() returns an id at once; wait(id) blocks."""
+
+    def __init__(self, **kw):
+        L = lib()
+        _bind_server(L)
+        cfg = ServerConfig()
+        L.jsvd_server_config_default(ctypes.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        h = _vp()
+        check(L.jsvd_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._shapes = {}
+
+    def submit(self, a, relerr: float) -> int:
+        arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        m, n = arr.shape
+        rid = ctypes.c_uint64()
+        check(lib().jsvd_server_submit(self._h, arr.ctypes.data, m, n, float(relerr), ctypes.byref(rid)))
+        self._shapes[rid.value] = (m, n)
+        return rid.value
+
+    def wait(self, rid: int):
+        m, n = self._shapes.pop(rid)
+        u = np.zeros((m, n))
+        s = np.zeros(n)
+        v = np.zeros((n, n))
+        info = RequestInfo()
+        st = lib().jsvd_server_wait(self._h, rid, u.ctypes.data, s.ctypes.data, v.ctypes.data, ctypes.byref(info))
+        check(st)
+        return u, s, v, info
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().jsvd_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            check(lib().jsvd_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,77 @@
+"""Request loop (continuous batching) on cuda:0.
+
+Requests of mixed shapes and precisions are submitted from a thread while earlier ones run; each result
+must equal, bit for bit, the result of the same request served alone by a fresh one-lane server (the
+kernels compute every request independently of its batch), and meet its accuracy: sorted sigma normwise
+error vs numpy (LAPACK gesdd) <= 1e-5 for relerr-1e-6 requests (FP32) and <= 1e-12 for relerr-1e-12
+requests (FP64).
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import jsvd
+from helpers import sorted_desc
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        m = int(np.exp(rng.uniform(np.log(16), np.log(600))))
+        n = int(rng.integers(4, min(m, 160) + 1))
+        kappa = 10.0 ** rng.uniform(1, 6)
+        q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+        q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        s = kappa ** (-np.arange(n) / max(n - 1, 1))
+        out.append(((q1 * s) @ q2.T, 1e-6 if i % 2 == 0 else 1e-12))
+    return out
+
+
+def test_server_mix_equals_alone_and_accuracy(torch_cuda):
+    reqs = _requests(48, 5)
+    srv = jsvd.Server(lanes=4, small_max_rows=256, small_max_columns=128)
+    ids = [None] * len(reqs)
+
+    def producer():
+        for i, (a, rel) in enumerate(reqs):
+            ids[i] = srv.submit(a, rel)
+            time.sleep(0.002)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    results = [srv.wait(i) for i in ids]
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == len(reqs) and stats["batches"] > 0 and stats["plans"] > 0
+    assert stats["max_in_flight"] > 1, "requests overlapped on the device"
+
+    alone = jsvd.Server(lanes=1)
+    for (a, rel), (u, s, v, info) in zip(reqs, results):
+        assert info.status == 0
+        rid = alone.submit(a, rel)
+        u1, s1, v1, info1 = alone.wait(rid)
+        assert info1.batch_size == 1
+        assert np.array_equal(s, s1) and np.array_equal(u, u1) and np.array_equal(v, v1)
+        ref = sorted_desc(np.linalg.svd(a, compute_uv=False))
+        err = np.max(np.abs(sorted_desc(s) - ref)) / ref[0]
+        assert err < (1e-5 if rel >= 1e-7 else 1e-12), (a.shape, rel, err)
+    alone.close()
+
+
+def test_server_budget_admission(torch_cuda):
+    reqs = _requests(12, 9)
+    biggest = max(8 * (3 * a.shape[0] * a.shape[1] + 2 * a.shape[1] ** 2 + a.shape[1]) for a, _ in reqs)
+    budget = 2 * biggest
+    srv = jsvd.Server(lanes=4, hbm_budget_bytes=budget)
+    ids = [srv.submit(a, rel) for a, rel in reqs]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert all(r[3].status == 0 for r in res)
+    assert st["budget_peak"] <= budget and st["budget_in_use"] == 0
