@@ -48,10 +48,17 @@ namespace jsvd
             J = dev::circle_at(Nb - 1 - slot, round, Nb);
         }
 
+        // Physical column of local column c (0..kW-1) of block pair (I, J) under the column map `perm`
+        // (logical column k of the sweep order is stored in physical column perm[k]).
+        __device__ inline int64_t pair_column(const int *__restrict__ perm, int I, int J, int c)
+        {
+            return perm[(c < kB) ? (I * kB + c) : (J * kB + c - kB)];
+        }
+
         // Loads rows [r0, r0+kH) of the kW columns [X_I X_J] into LDS as xs[c][r] (column-major, padded).
         template <typename T>
-        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, int I, int J, int64_t r0,
-                                         T (*xs)[kH + 1])
+        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, const int *__restrict__ perm, int I,
+                                         int J, int64_t r0, T (*xs)[kH + 1])
         {
             // kW * kH elements, 256 threads: each thread moves kW*kH/256 elements; consecutive threads read
             // consecutive rows of one column (coalesced).  All loads are issued before the LDS stores so
@@ -64,8 +71,7 @@ namespace jsvd
                 const int e = threadIdx.x + 256 * k;
                 const int c = e / kH;
                 const int r = e % kH;
-                const int64_t col = (c < kB) ? (static_cast<int64_t>(I) * kB + c) : (static_cast<int64_t>(J) * kB + c - kB);
-                tmp[k] = x[col * ld + r0 + r];
+                tmp[k] = x[pair_column(perm, I, J, c) * ld + r0 + r];
             }
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
@@ -76,7 +82,8 @@ namespace jsvd
         }
 
         template <typename T>
-        __global__ __launch_bounds__(256) void block_gram_kernel(const T *__restrict__ a, int64_t lda, int nb, int Nb,
+        __global__ __launch_bounds__(256) void block_gram_kernel(const T *__restrict__ a, int64_t lda,
+                                                                 const int *__restrict__ perm, int nb, int Nb,
                                                                  int round, int slabs, T *__restrict__ partial,
                                                                  const int *__restrict__ skip_if_zero)
         {
@@ -93,7 +100,7 @@ namespace jsvd
             {
                 return;
             }
-            load_slab(a, lda, I, J, static_cast<int64_t>(s) * kH, xs);
+            load_slab(a, lda, perm, I, J, static_cast<int64_t>(s) * kH, xs);
             __syncthreads();
             const int lane = threadIdx.x % 64;
             const int wave = threadIdx.x / 64;
@@ -143,7 +150,7 @@ namespace jsvd
         __global__ __launch_bounds__(kEigenThreads) void block_eigen_kernel(
             const T *__restrict__ partial, int slabs, int nb, int Nb, int round, double eps, T *__restrict__ q_out,
             int *__restrict__ apply_flag, int *__restrict__ rot_count, unsigned long long *__restrict__ offmax,
-            const int *__restrict__ skip_if_zero)
+            const int *__restrict__ perm, double *__restrict__ norm2, const int *__restrict__ skip_if_zero)
         {
             __shared__ T g[kW][kW + 1];
             __shared__ T q[kW][kW + 1];
@@ -315,6 +322,12 @@ namespace jsvd
             {
                 qo[e] = q[e / kW][e % kW]; // row-major Q[c][j]
             }
+            // squared norms of the pair's columns after the rotation (the diagonal of Q^T G Q), for the
+            // norm ordering of the next sweep
+            if (norm2 != nullptr && tid < kW)
+            {
+                norm2[pair_column(perm, I, J, tid)] = static_cast<double>(g[tid][tid]);
+            }
             if (tid == 0)
             {
                 apply_flag[slot] = any;
@@ -340,6 +353,7 @@ namespace jsvd
                                                                   T *__restrict__ v, int64_t ldv, int nb, int Nb,
                                                                   int round, const T *__restrict__ q_in,
                                                                   const int *__restrict__ apply_flag,
+                                                                  const int *__restrict__ perm,
                                                                   const int *__restrict__ skip_if_zero)
         {
             __shared__ T xs[kW][kH + 1];
@@ -359,7 +373,7 @@ namespace jsvd
             T *x = (sy < slabs_a) ? a : v;
             const int64_t ld = (sy < slabs_a) ? lda : ldv;
             const int64_t r0 = static_cast<int64_t>((sy < slabs_a) ? sy : sy - slabs_a) * kH;
-            load_slab(x, ld, I, J, r0, xs);
+            load_slab(x, ld, perm, I, J, r0, xs);
             const T *qg = q_in + static_cast<int64_t>(slot) * (kW * kW);
             for (int e = threadIdx.x; e < kW * kW; e += 256)
             {
@@ -411,8 +425,7 @@ namespace jsvd
             {
                 const int c = e / kH;
                 const int r = e % kH;
-                const int64_t col = (c < kB) ? (static_cast<int64_t>(I) * kB + c) : (static_cast<int64_t>(J) * kB + c - kB);
-                x[col * ld + r0 + r] = xs[c][r];
+                x[pair_column(perm, I, J, c) * ld + r0 + r] = xs[c][r];
             }
         }
 
@@ -461,30 +474,114 @@ namespace jsvd
             }
             v[i] = (i / ldv == i % ldv) ? T(1) : T(0);
         }
+        // Column order of the next sweep: physical columns sorted by descending squared norm (ties by column
+        // index), one workgroup, bitonic network over the next power of two >= n_pad in LDS.
+        __global__ __launch_bounds__(1024) void column_order_kernel(const double *__restrict__ norm2, int n_pad,
+                                                                    int *__restrict__ perm,
+                                                                    const int *__restrict__ skip_if_zero)
+        {
+            __shared__ double key[kSortMax];
+            __shared__ int val[kSortMax];
+            if (skip_if_zero != nullptr && *skip_if_zero == 0)
+            {
+                return;
+            }
+            int size = 1;
+            while (size < n_pad)
+            {
+                size *= 2;
+            }
+            for (int i = threadIdx.x; i < size; i += 1024)
+            {
+                key[i] = (i < n_pad) ? norm2[i] : -1.0;
+                val[i] = i;
+            }
+            __syncthreads();
+            for (int k = 2; k <= size; k *= 2)
+            {
+                for (int j = k / 2; j > 0; j /= 2)
+                {
+                    for (int i = threadIdx.x; i < size; i += 1024)
+                    {
+                        const int x = i ^ j;
+                        if (x > i)
+                        {
+                            // does element i belong before element x in the final (descending) order?
+                            const bool i_first = key[i] > key[x] || (key[i] == key[x] && val[i] < val[x]);
+                            const bool ascending_block = (i & k) == 0;
+                            if (ascending_block != i_first)
+                            {
+                                const double tk = key[i];
+                                key[i] = key[x];
+                                key[x] = tk;
+                                const int tv = val[i];
+                                val[i] = val[x];
+                                val[x] = tv;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = threadIdx.x; i < n_pad; i += 1024)
+            {
+                perm[i] = val[i];
+            }
+        }
+
+        __global__ void iota_kernel(int *__restrict__ perm, int64_t n)
+        {
+            const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i < n)
+            {
+                perm[i] = static_cast<int>(i);
+            }
+        }
     } // namespace
 
     template <typename T>
     void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
-                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
-                            hipStream_t stream)
+                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
+                            double *norm2, const int *skip_if_zero, hipStream_t stream)
     {
         const int pairs = g.Nb / 2;
-        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, g.slabs_a), dim3(256), 0, stream, a, g.lda, g.nb, g.Nb,
-                           round, g.slabs_a, partial, skip_if_zero);
+        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, g.slabs_a), dim3(256), 0, stream,
+                           static_cast<const T *>(a), g.lda, perm, g.nb, g.Nb, round, g.slabs_a, partial, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream, partial, g.slabs_a, g.nb, g.Nb,
-                           round, eps, q, apply_flag, rot_count, offmax, skip_if_zero);
+        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream,
+                           static_cast<const T *>(partial), g.slabs_a, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
+                           offmax, perm, norm2, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, g.slabs_a + g.slabs_v), dim3(256), 0, stream, a, g.lda,
                            g.slabs_a, v, g.ldv, g.nb, g.Nb, round, static_cast<const T *>(q),
-                           static_cast<const int *>(apply_flag), skip_if_zero);
+                           static_cast<const int *>(apply_flag), perm, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
     }
 
     template void launch_block_round<float>(const BlockGeometry &, float *, float *, int, double, float *, float *,
-                                            int *, int *, unsigned long long *, const int *, hipStream_t);
+                                            int *, int *, unsigned long long *, const int *, double *, const int *,
+                                            hipStream_t);
     template void launch_block_round<double>(const BlockGeometry &, double *, double *, int, double, double *,
-                                             double *, int *, int *, unsigned long long *, const int *, hipStream_t);
+                                             double *, int *, int *, unsigned long long *, const int *, double *,
+                                             const int *, hipStream_t);
+
+    void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
+                             hipStream_t stream)
+    {
+        if (n_pad > kSortMax)
+        {
+            return; // beyond one workgroup's LDS: keep the current order
+        }
+        hipLaunchKernelGGL(column_order_kernel, dim3(1), dim3(1024), 0, stream, norm2, static_cast<int>(n_pad), perm,
+                           skip_if_zero);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    void launch_iota(int *perm, int64_t n, hipStream_t stream)
+    {
+        hipLaunchKernelGGL(iota_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, perm, n);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
 
     void launch_convert_f64_to_f32(const double *src, float *dst, int64_t count, hipStream_t stream)
     {

@@ -43,10 +43,20 @@ namespace jsvd
 
     // One block round (Gram partials, reduction + inner sweep, apply to A and V).  partial holds
     // pairs * slabs_a * kW * kW elements, q pairs * kW * kW, apply_flag pairs ints.
+    // perm maps the sweep's logical column order to physical columns (A and V columns are addressed
+    // through it); norm2 receives the squared norm of every column after its last rotation of the sweep.
     template <typename T>
     void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
-                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
-                            hipStream_t stream);
+                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
+                            double *norm2, const int *skip_if_zero, hipStream_t stream);
+
+    // Largest padded column count the one-workgroup ordering kernel sorts (LDS bound).
+    constexpr int kSortMax = 8192;
+
+    // perm <- physical columns by descending norm2 (no-op when *skip_if_zero == 0 or n_pad > kSortMax).
+    void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
+                             hipStream_t stream);
+    void launch_iota(int *perm, int64_t n, hipStream_t stream);
 
     void launch_convert_f64_to_f32(const double *src, float *dst, int64_t count, hipStream_t stream);
     void launch_convert_f32_to_f64(const float *src, double *dst, int64_t count, hipStream_t stream);

@@ -58,6 +58,8 @@ namespace jsvd
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
         JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * g_.pairs() * kW * kW));
         JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * g_.pairs()));
+        JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * g_.n_pad));
+        JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * g_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&rot_, sizeof(int) * 2 * cap_));
         JSVD_HIP_CHECK(hipMalloc(&off_, sizeof(unsigned long long) * 2 * cap_));
         JSVD_HIP_CHECK(hipHostMalloc(&h_rot_, sizeof(int) * 2 * cap_, hipHostMallocDefault));
@@ -92,7 +94,8 @@ namespace jsvd
         for (void *p : {static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
-                        static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(off_)})
+                        static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(off_),
+                        static_cast<void *>(perm_), static_cast<void *>(norm2_)})
         {
             (void)hipFree(p);
         }
@@ -119,12 +122,21 @@ namespace jsvd
         T *partial = reinterpret_cast<T *>(partial_);
         T *q = reinterpret_cast<T *>(q_);
         int enqueued = 0;
+        // Sweep 0 runs in the incoming column order; every later sweep first orders the columns by descending
+        // norm (the norms left by the previous sweep), which groups columns of similar size into the same
+        // blocks and cuts the sweep count.  A and V columns are addressed through perm_, nothing moves.
+        launch_iota(perm_, g_.n_pad, stream_);
         for (int s = 0; s < max_sweeps; ++s)
         {
             const int *skip = (s > 0) ? rot + (s - 1) : nullptr;
+            if (s > 0)
+            {
+                launch_column_order(norm2_, g_.n_pad, perm_, skip, stream_);
+            }
             for (int r = 0; r < g_.rounds(); ++r)
             {
-                launch_block_round<T>(g_, a, v, r, eps, partial, q, flags_, rot + s, off + s, skip, stream_);
+                launch_block_round<T>(g_, a, v, r, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip,
+                                      stream_);
             }
             JSVD_HIP_CHECK(hipMemcpyAsync(h + s, rot + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
             JSVD_HIP_CHECK(hipEventRecord(done_[s & 1], stream_));

@@ -43,6 +43,8 @@ namespace jsvd
         double *partial_ = nullptr; // reused as float for the FP32 phase
         double *q_ = nullptr;
         int *flags_ = nullptr;
+        int *perm_ = nullptr;      // [n_pad] logical -> physical column of the current sweep
+        double *norm2_ = nullptr;  // [n_pad] squared column norms after each column's last rotation
         int *rot_ = nullptr;                 // [2 * cap] rotation counts (FP32 sweeps, then FP64 sweeps)
         unsigned long long *off_ = nullptr;  // [2 * cap]
         int *h_rot_ = nullptr;               // pinned mirror
