@@ -81,10 +81,13 @@ namespace jsvd
             }
         }
 
+        // Partial Gram of row slabs [ps * per, min(slabs, (ps + 1) * per)) of block pair `slot`; the partials of a
+        // pair are partial[slot][0 .. parts).
         template <typename T>
         __global__ __launch_bounds__(256) void block_gram_kernel(const T *__restrict__ a, int64_t lda,
                                                                  const int *__restrict__ perm, int nb, int Nb,
-                                                                 int round, int slabs, T *__restrict__ partial,
+                                                                 int round, int slabs, int per, int parts,
+                                                                 T *__restrict__ partial,
                                                                  const int *__restrict__ skip_if_zero)
         {
             __shared__ T xs[kW][kH + 1];
@@ -93,15 +96,13 @@ namespace jsvd
                 return;
             }
             const int slot = blockIdx.x;
-            const int s = blockIdx.y;
+            const int ps = blockIdx.y;
             int I, J;
             block_pair(slot, round, Nb, I, J);
             if (I >= nb || J >= nb)
             {
                 return;
             }
-            load_slab(a, lda, perm, I, J, static_cast<int64_t>(s) * kH, xs);
-            __syncthreads();
             const int lane = threadIdx.x % 64;
             const int wave = threadIdx.x / 64;
             const int i0 = 16 * wave; // this wave computes G rows i0..i0+15, all kW columns
@@ -110,18 +111,28 @@ namespace jsvd
             {
                 acc[t] = typename Mfma<T>::Acc{0, 0, 0, 0};
             }
-            for (int r0 = 0; r0 < kH; r0 += 4)
+            const int s_end = min(slabs, (ps + 1) * per);
+            for (int s = ps * per; s < s_end; ++s)
             {
-                const int kr = r0 + lane / 16;
-                const T av = xs[i0 + lane % 16][kr];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
+                if (s > ps * per)
                 {
-                    const T bv = xs[16 * t + lane % 16][kr];
-                    acc[t] = Mfma<T>::step(av, bv, acc[t]);
+                    __syncthreads(); // every wave is done with the previous slab
+                }
+                load_slab(a, lda, perm, I, J, static_cast<int64_t>(s) * kH, xs);
+                __syncthreads();
+                for (int r0 = 0; r0 < kH; r0 += 4)
+                {
+                    const int kr = r0 + lane / 16;
+                    const T av = xs[i0 + lane % 16][kr];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                    {
+                        const T bv = xs[16 * t + lane % 16][kr];
+                        acc[t] = Mfma<T>::step(av, bv, acc[t]);
+                    }
                 }
             }
-            T *out = partial + (static_cast<int64_t>(slot) * slabs + s) * (kW * kW);
+            T *out = partial + (static_cast<int64_t>(slot) * parts + ps) * (kW * kW);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
             {
@@ -138,13 +149,82 @@ namespace jsvd
         // Reduction of the slab partials and one inner cyclic Jacobi sweep in Gram space.
         //
         // G is symmetric and kept only in its upper triangle (g[min][max]).  Each inner round pairs the 64
-        // columns by the round-robin circle; thread t < 32 forms the rotation of pair t in FP64, then the
-        // 528 2x2 blocks (pair t rows, pair u columns, t <= u) get the two-sided update J_t^T G J_u and the
-        // 2048 (row, pair) entries of Q the one-sided update Q J_u.  Thread tid < 528 owns 2x2 block tid of
-        // the upper triangle for the whole sweep; the column indices of a round come from the closed form of
-        // the circle.  A block pair none of whose 2016 column pairs passes the skip test is left untouched.
-        constexpr int kEigenThreads = 1024; // one thread per 2x2 block (528) and per Q row pair (2 x 1024)
-        constexpr int kTriBlocks = (kW / 2) * (kW / 2 + 1) / 2;
+        // columns by the round-robin circle (pair u of a round = circle positions u and 63 - u).  A round has
+        // two phases separated by a barrier:
+        //   phase 1: lanes 0..31 of wave 0 form the 32 rotations of the round in FP64 (reference skip test) and
+        //            publish them; threads 0..527 load their 2x2 block of G (block (t, u), t <= u, of the 32 x 32
+        //            pair grid); the Q waves apply the previous round's rotations to Q (off the critical path);
+        //   phase 2: threads 0..527 write J_t^T G J_u back in place.
+        // The instruction issue of the 16 waves bounds a round, so the work is packed into as few waves as it
+        // needs: G blocks on waves 0..8, Q on waves 12..15.  Q is kept in registers in circle-position order:
+        // thread (row k, segment s) of the Q waves holds Q[k][col(p)] for the positions p = 8s..8s+7 and their
+        // partners 63-p, so every rotation of a round is local to a thread; after a round the positions move by
+        // one along the circle (1..63 cyclically, 0 fixed), which moves one value to each neighbouring segment.
+        // Column indices advance incrementally (col(p, r+1) = col(p, r) - 1 on 1..63).  A block pair none of
+        // whose 2016 column pairs passes the skip test is left untouched.
+        constexpr int kEigenThreads = 1024;
+        constexpr int kPairs = kW / 2;
+        constexpr int kTriBlocks = kPairs * (kPairs + 1) / 2;   // 2x2 blocks of the upper triangle (528)
+        constexpr int kQThreads = 256;                          // 64 rows x 4 segments of 8 position pairs
+        constexpr int kQFirst = kEigenThreads - kQThreads;      // Q waves are the last four
+        static_assert(kW == 64 && kTriBlocks <= kQFirst, "eigen thread roles assume 64 columns");
+
+        // Column at a circle position in the next inner round.
+        __device__ inline int next_column(int c)
+        {
+            return c == 0 ? 0 : (c == 1 ? kW - 1 : c - 1);
+        }
+
+        // One inner round on a thread's Q registers: rotate its 8 position pairs, then move the positions one step
+        // along the circle (values crossing a segment boundary come from the neighbouring lanes).
+        template <typename T>
+        __device__ inline void q_round(T (&lo)[8], T (&hi)[8], const T (*cs)[2], int seg)
+        {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+            {
+                const T c = cs[8 * seg + i][0], s = cs[8 * seg + i][1];
+                const T x = lo[i], z = hi[i];
+                lo[i] = c * x - s * z;
+                hi[i] = s * x + c * z;
+            }
+            const T out_lo = lo[7]; // position 8s+7 -> 8s+8 (next segment; segment 3: position 32, own hi[7])
+            const T out_hi = hi[0]; // position 63-8s -> 64-8s (previous segment; segment 0: position 1, own lo[1])
+            const T from_prev = __shfl_up(out_lo, 1, 64);
+            const T from_next = __shfl_down(out_hi, 1, 64);
+            const T lo0 = lo[0];
+#pragma unroll
+            for (int i = 7; i >= 1; --i)
+            {
+                lo[i] = lo[i - 1];
+            }
+#pragma unroll
+            for (int i = 0; i < 7; ++i)
+            {
+                hi[i] = hi[i + 1];
+            }
+            lo[0] = (seg == 0) ? lo0 : from_prev; // position 0 never moves
+            hi[7] = (seg == 3) ? out_lo : from_next;
+            if (seg == 0)
+            {
+                lo[1] = out_hi; // position 63 -> 1
+            }
+        }
+
+        template <typename T>
+        __device__ inline bool block_rotation(double app, double aqq, double apq, double eps, double &c, double &s)
+        {
+            // The FP32 phase applies its rotations in FP32, so the shorter FP64 chain is exact enough there; the
+            // FP64 phase keeps the division form.
+            if constexpr (sizeof(T) == 4)
+            {
+                return dev::rotation_params_fast(app, aqq, apq, eps, c, s);
+            }
+            else
+            {
+                return dev::rotation_params_block(app, aqq, apq, eps, c, s);
+            }
+        }
 
         template <typename T>
         __global__ __launch_bounds__(kEigenThreads) void block_eigen_kernel(
@@ -153,9 +233,8 @@ namespace jsvd
             const int *__restrict__ perm, double *__restrict__ norm2, const int *__restrict__ skip_if_zero)
         {
             __shared__ T g[kW][kW + 1];
-            __shared__ T q[kW][kW + 1];
-            __shared__ double cs[kW / 2][2];
-            __shared__ int rotated[kW / 2];
+            __shared__ T cs[2][kPairs][2];              // rotations of the current and the previous round
+            __shared__ unsigned long long rmask[2];     // bit u: pair u rotates
             __shared__ double red[kEigenThreads / 64];
             __shared__ int any;
             __shared__ int needed;
@@ -171,14 +250,14 @@ namespace jsvd
                 return;
             }
             const int tid = threadIdx.x;
-            // This thread's 2x2 block (t, u), t <= u, of the upper triangle of the 32 x 32 block grid.
+            // This thread's 2x2 block (t, u), t <= u, of the upper triangle of the 32 x 32 pair grid.
             int bt = 0, bu = 0;
             if (tid < kTriBlocks)
             {
                 int rem = tid;
-                while (rem >= kW / 2 - bt)
+                while (rem >= kPairs - bt)
                 {
-                    rem -= kW / 2 - bt;
+                    rem -= kPairs - bt;
                     ++bt;
                 }
                 bu = bt + rem;
@@ -213,12 +292,22 @@ namespace jsvd
                     }
                     g[i][j] = acc;
                 }
-                q[i][j] = (i == j) ? T(1) : T(0);
             }
             if (tid == 0)
             {
                 any = 0;
                 needed = 0;
+            }
+            // Q = I in circle-position order (round 0 positions).
+            const bool qthr = tid >= kQFirst;
+            const int qk = (tid - kQFirst) / 4;
+            const int seg = (tid - kQFirst) % 4;
+            T qlo[8], qhi[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+            {
+                qlo[i] = (dev::circle_at_fast(8 * seg + i, 0, kW) == qk) ? T(1) : T(0);
+                qhi[i] = (dev::circle_at_fast(kW - 1 - 8 * seg - i, 0, kW) == qk) ? T(1) : T(0);
             }
             __syncthreads();
             // Off-orthogonality of the pair's 64 columns before the sweep (convergence measure), and whether
@@ -251,76 +340,98 @@ namespace jsvd
             }
             __syncthreads();
             int my_rotations = 0;
-            for (int rr = 0; needed && rr < kW - 1; ++rr)
+            const bool gthr = tid < kTriBlocks;
+            int x1 = dev::circle_at_fast(bt, 0, kW), x2 = dev::circle_at_fast(kW - 1 - bt, 0, kW);
+            int z1 = dev::circle_at_fast(bu, 0, kW), z2 = dev::circle_at_fast(kW - 1 - bu, 0, kW);
+            const int pu = tid % kPairs; // wave 0: the pair this lane rotates
+            int p1 = dev::circle_at_fast(pu, 0, kW), p2 = dev::circle_at_fast(kW - 1 - pu, 0, kW);
+            const bool run = needed != 0;
+            for (int rr = 0; run && rr < kW - 1; ++rr)
             {
-                if (tid < kW / 2)
+                const int b = rr & 1;
+                // phase 1: block loads, the round's rotations, the previous round's Q update
+                T g11 = 0, g12 = 0, g21 = 0, g22 = 0;
+                int o11 = 0, o12 = 0, o21 = 0, o22 = 0;
+                T *gf = &g[0][0];
+                if (gthr)
                 {
-                    const int p = dev::circle_at_fast(tid, rr, kW);
-                    const int qq = dev::circle_at_fast(kW - 1 - tid, rr, kW);
+                    o11 = min(x1, z1) * (kW + 1) + max(x1, z1);
+                    o12 = min(x1, z2) * (kW + 1) + max(x1, z2);
+                    o21 = min(x2, z1) * (kW + 1) + max(x2, z1);
+                    o22 = min(x2, z2) * (kW + 1) + max(x2, z2);
+                    g11 = gf[o11];
+                    g12 = gf[o12];
+                    g21 = gf[o21];
+                    g22 = gf[o22];
+                }
+                if (tid < 64)
+                {
                     double c, s;
-                    const bool rot = dev::rotation_params_block(static_cast<double>(g[p][p]),
-                                                                static_cast<double>(g[qq][qq]),
-                                                                static_cast<double>(g[min(p, qq)][max(p, qq)]), eps, c, s);
-                    cs[tid][0] = c;
-                    cs[tid][1] = s;
-                    rotated[tid] = rot ? 1 : 0;
-                    my_rotations += rot ? 1 : 0;
+                    const bool rot = block_rotation<T>(static_cast<double>(g[p1][p1]), static_cast<double>(g[p2][p2]),
+                                                       static_cast<double>(g[min(p1, p2)][max(p1, p2)]), eps, c, s);
+                    const unsigned long long m = __ballot(rot);
+                    if (tid < kPairs)
+                    {
+                        cs[b][tid][0] = static_cast<T>(c);
+                        cs[b][tid][1] = static_cast<T>(s);
+                        my_rotations += rot ? 1 : 0;
+                    }
+                    if (tid == 0)
+                    {
+                        rmask[b] = m;
+                    }
+                }
+                if (qthr && rr > 0)
+                {
+                    q_round<T>(qlo, qhi, cs[b ^ 1], seg);
                 }
                 __syncthreads();
-                if (tid < kTriBlocks && (rotated[bt] || rotated[bu]))
+                // phase 2: J_t^T G J_u in place (rows by J_t^T, x' = c x - s y on row pairs, then columns by J_u)
+                if (gthr)
                 {
-                    const int x1 = dev::circle_at_fast(bt, rr, kW), x2 = dev::circle_at_fast(kW - 1 - bt, rr, kW);
-                    const int y1 = dev::circle_at_fast(bu, rr, kW), y2 = dev::circle_at_fast(kW - 1 - bu, rr, kW);
-                    const T ct = static_cast<T>(cs[bt][0]), st = static_cast<T>(cs[bt][1]);
-                    const T cu = static_cast<T>(cs[bu][0]), su = static_cast<T>(cs[bu][1]);
-                    T &e11 = g[min(x1, y1)][max(x1, y1)];
-                    T &e12 = g[min(x1, y2)][max(x1, y2)];
-                    T &e21 = g[min(x2, y1)][max(x2, y1)];
-                    T &e22 = g[min(x2, y2)][max(x2, y2)];
-                    const T g11 = e11, g12 = e12, g21 = e21, g22 = e22;
-                    // rows by J_t^T (x' = c x - s y on row pairs), then columns by J_u
-                    const T h11 = ct * g11 - st * g21, h12 = ct * g12 - st * g22;
-                    const T h21 = st * g11 + ct * g21, h22 = st * g12 + ct * g22;
-                    const T n11 = cu * h11 - su * h12;
-                    const T n12 = su * h11 + cu * h12;
-                    const T n21 = cu * h21 - su * h22;
-                    const T n22 = su * h21 + cu * h22;
-                    e11 = n11;
-                    e12 = n12;
-                    e22 = n22;
-                    if (bt != bu)
+                    const unsigned long long m = rmask[b];
+                    if (((m >> bt) | (m >> bu)) & 1ull)
                     {
-                        e21 = n21; // on the diagonal block e21 aliases e12 (one symmetric entry)
+                        const T ct = cs[b][bt][0], st = cs[b][bt][1];
+                        const T cu = cs[b][bu][0], su = cs[b][bu][1];
+                        const T h11 = ct * g11 - st * g21, h12 = ct * g12 - st * g22;
+                        const T h21 = st * g11 + ct * g21, h22 = st * g12 + ct * g22;
+                        gf[o11] = cu * h11 - su * h12;
+                        gf[o12] = su * h11 + cu * h12;
+                        gf[o22] = su * h21 + cu * h22;
+                        if (bt != bu)
+                        {
+                            gf[o21] = cu * h21 - su * h22; // on the diagonal block entry 21 aliases entry 12
+                        }
                     }
+                    x1 = next_column(x1);
+                    x2 = next_column(x2);
+                    z1 = next_column(z1);
+                    z2 = next_column(z2);
                 }
-                {
-                    // Q J_u: thread handles pair u = tid % 32 on rows k0 = tid / 32 and k0 + 32 (the loads of
-                    // both rows are issued before the stores).
-                    const int u = tid % (kW / 2);
-                    const int k0 = tid / (kW / 2);
-                    if (rotated[u])
-                    {
-                        const int y1 = dev::circle_at_fast(u, rr, kW), y2 = dev::circle_at_fast(kW - 1 - u, rr, kW);
-                        const T cu = static_cast<T>(cs[u][0]), su = static_cast<T>(cs[u][1]);
-                        const T x0 = q[k0][y1], z0 = q[k0][y2];
-                        const T x1 = q[k0 + kW / 2][y1], z1 = q[k0 + kW / 2][y2];
-                        q[k0][y1] = cu * x0 - su * z0;
-                        q[k0][y2] = su * x0 + cu * z0;
-                        q[k0 + kW / 2][y1] = cu * x1 - su * z1;
-                        q[k0 + kW / 2][y2] = su * x1 + cu * z1;
-                    }
-                }
+                p1 = next_column(p1);
+                p2 = next_column(p2);
                 __syncthreads();
             }
-            if (tid < kW / 2 && my_rotations > 0)
+            if (run && qthr)
+            {
+                q_round<T>(qlo, qhi, cs[(kW - 2) & 1], seg); // the last round's update
+            }
+            if (tid < kPairs && my_rotations > 0)
             {
                 atomicAdd(&any, my_rotations);
             }
             __syncthreads();
+            // After kW - 1 moves the circle is back at its round-0 positions.
             T *qo = q_out + static_cast<int64_t>(slot) * (kW * kW);
-            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            if (qthr)
             {
-                qo[e] = q[e / kW][e % kW]; // row-major Q[c][j]
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                {
+                    qo[qk * kW + dev::circle_at_fast(8 * seg + i, 0, kW)] = qlo[i]; // row-major Q[c][j]
+                    qo[qk * kW + dev::circle_at_fast(kW - 1 - 8 * seg - i, 0, kW)] = qhi[i];
+                }
             }
             // squared norms of the pair's columns after the rotation (the diagonal of Q^T G Q), for the
             // norm ordering of the next sweep
@@ -545,11 +656,19 @@ namespace jsvd
                             double *norm2, const int *skip_if_zero, hipStream_t stream)
     {
         const int pairs = g.Nb / 2;
-        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, g.slabs_a), dim3(256), 0, stream,
-                           static_cast<const T *>(a), g.lda, perm, g.nb, g.Nb, round, g.slabs_a, partial, skip_if_zero);
+        // Slabs per Gram workgroup: fewer partials for the eigen kernel to reduce (its reduction runs on one CU
+        // per pair) while the Gram grid still covers the chip.
+        int per = 1;
+        while ((g.slabs_a + per - 1) / per > 8 && pairs * ((g.slabs_a + 2 * per - 1) / (2 * per)) >= 256)
+        {
+            per *= 2;
+        }
+        const int parts = (g.slabs_a + per - 1) / per;
+        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, parts), dim3(256), 0, stream, static_cast<const T *>(a),
+                           g.lda, perm, g.nb, g.Nb, round, g.slabs_a, per, parts, partial, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream,
-                           static_cast<const T *>(partial), g.slabs_a, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
+                           static_cast<const T *>(partial), parts, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
                            offmax, perm, norm2, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, g.slabs_a + g.slabs_v), dim3(256), 0, stream, a, g.lda,

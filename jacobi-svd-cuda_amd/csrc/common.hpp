@@ -166,6 +166,28 @@ namespace jsvd::dev
         return false;
     }
 
+    // rotation_params_block with a shorter chain: d and g are scaled by an exact power of two (no division),
+    // and with w = 1 / sqrt(den^2 + g^2) the rotation is c = den w, s = sign(d) g w (s = t c), so the chain is
+    // one sqrt and one rsqrt.  Branch-free: the parameters of a skipped pair are replaced by (1, 0).
+    __device__ inline bool rotation_params_fast(double app, double aqq, double apq, double eps, double &c,
+                                                double &s)
+    {
+        const double rhs = eps * sqrt(fmax(app, 0.0) * fmax(aqq, 0.0));
+        const bool rot = fabs(apq) > rhs && fabs(apq) > 1.0e-300;
+        const double d0 = aqq - app;
+        const double g0 = 2.0 * apq;
+        int e = 0;
+        (void)frexp(fmax(fabs(d0), fabs(g0)), &e);
+        const double d = ldexp(d0, -e);
+        const double g = ldexp(g0, -e);
+        const double hyp = sqrt(d * d + g * g);
+        const double den = fabs(d) + hyp;
+        const double w = rsqrt(den * den + g * g);
+        c = rot ? den * w : 1.0;
+        s = rot ? ((d >= 0.0) ? g : -g) * w : 0.0;
+        return rot;
+    }
+
     // Off-orthogonality of a pair, |gamma| / sqrt(alpha beta), 0 when a column is zero.
     __host__ __device__ inline double pair_cosine(double app, double aqq, double apq)
     {
