@@ -4,9 +4,11 @@ Default workload (one step = one full SVD solve, input resident in HBM, outputs 
   tall_geometric 4096 x 1024, A = Q1 diag(sigma) Q2^T with sigma geometric from 1 to 1e-8,
   plain FP64 cyclic one-sided Jacobi, eps 1e-14, at most 60 sweeps.
 
-Prints ONE JSON line with the metric, the roofline of the dominant kernel (the fused round kernel,
-timed with HIP events on the stream it runs on) and a CPU baseline (the oracle's C restatement of the
-reference algorithm, OpenMP over the pairs of a round, on a bounded sample of sweeps).
+The headline value is the mixed FP32-MFMA + FP64 path; the plain FP64 path is timed beside it.  Prints ONE
+JSON line with the metric, the roofline of the mixed path's dominant kernel (block_eigen_kernel<float>) and of
+the plain path's fused round kernel (both timed with HIP events on the stream they run on), and a CPU baseline
+(the oracle's C restatement of the reference algorithm, OpenMP over the pairs of a round, on a bounded sample
+of sweeps).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tall|plain] [--no-cpu-baseline]
 """
@@ -43,9 +45,60 @@ def make_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+# Flops of one Gram-space inner sweep of a 64-column block pair: 63 rounds, each updating the 528 2x2 blocks
+# of the upper triangle two-sidedly (24 flops each) and the 64 x 32 (row, pair) entries of Q (6 flops each).
+EIGEN_FLOPS_PER_PAIR = 63 * (528 * 24 + 64 * 32 * 6)
+
+
+def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
+    """The mixed path's dominant kernel, block_eigen_kernel<float>, on the tall workload's round shape (16 block
+    pairs of 64 columns per round): Grams of column-graded N(0,1) 4096 x 64 blocks, every pair rotating.
+    Timed with two HIP events around `launches` back-to-back launches on the stream they run on."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    x = torch.randn((pairs, m, 64), dtype=torch.float64, device="cuda", generator=gen)
+    x *= torch.pow(10.0, -4.0 * torch.arange(64, device="cuda", dtype=torch.float64) / 63.0)
+    gram = torch.matmul(x.transpose(1, 2).contiguous(), x).to(torch.float32).contiguous()
+    q = torch.empty_like(gram)
+    rot = torch.zeros(pairs, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream)   # warm the code path
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(launches):
+        jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    avg_us = 1000.0 * t0.elapsed_time(t1) / launches
+    flops = pairs * EIGEN_FLOPS_PER_PAIR
+    achieved = flops / (avg_us * 1e-6) / 1e12
+    return {
+        "kernel": "block_eigen_kernel<float> (Gram-space inner sweep of a block round: 63 rounds of 2x2 updates "
+                  "of G and Q, rotation parameters in FP64; one workgroup per block pair)",
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": None,
+        "avg_launch_us": avg_us,
+        "algorithmic_flops_per_launch": flops,
+        "pairs_per_launch": pairs,
+        "rotated_pairs_per_block_pair": int(rot.float().mean().item()),
+        "note": "issue-bound on vector instructions (index arithmetic and 2x2 updates), far from either roof; "
+                "counted against the FP32 matrix peak",
+    }
+
+
 def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
-    """Times the fused round kernel with HIP events around each launch on the workload's own data
-    (first sweep: every pair rotates).  Returns (avg_us, algorithmic bytes per launch)."""
+    """Times the fused round kernel on the workload's own data (rounds 1..255 of the first sweep: every pair
+    rotates) with two HIP events around `launches` back-to-back launches (jsvd_rounds_f64, launched from C++ so
+    the queue stays ahead of the GPU) on the stream they run on.
+    Returns (avg_us, algorithmic bytes per launch, rotated pairs, pairs)."""
     m, n = a_host.shape
     lda, ldv = m + (m % 2), n + (n % 2)
     stream = torch.cuda.current_stream()
@@ -57,15 +110,17 @@ def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
     N = n + (n % 2)
     rounds = N - 1
     launches = min(launches, rounds)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    for r in range(launches):
-        starts[r].record(stream)
-        jsvd.round_(acol, lda, m, vcol, ldv, n, r, eps, rot_count=rot, stream=stream)
-        ends[r].record(stream)
+    jsvd.round_(acol, lda, m, vcol, ldv, n, 0, eps, stream=stream)   # warm the code path (rotates round 0)
+    rot.zero_()
     torch.cuda.synchronize()
-    total_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends))
+    t0.record(stream)
+    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, eps, rot_count=rot, stream=stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t0.elapsed_time(t1)
     rotated = int(rot.item())
     pairs = launches * (n // 2)
     # fused kernel: every pair reads its two A columns (Gram); a rotated pair also writes them and
@@ -85,16 +140,15 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
     vcol = torch.eye(n, dtype=torch.float64, device="cuda")
     rot = torch.zeros(1, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    starts = [torch.cuda.Event(enable_timing=True)]
+    ends = [torch.cuda.Event(enable_timing=True)]
     jsvd.round_(acol, lda, m, vcol, ldv, n, 0, 1e-15, stream=stream)   # warm the code path
     torch.cuda.synchronize()
-    for r in range(launches):
-        starts[r].record(stream)
-        jsvd.round_(acol, lda, m, vcol, ldv, n, r + 1, 1e-15, rot_count=rot, stream=stream)
-        ends[r].record(stream)
+    starts[0].record(stream)
+    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, 1e-15, rot_count=rot, stream=stream)
+    ends[0].record(stream)
     torch.cuda.synchronize()
-    total_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends))
+    total_ms = starts[0].elapsed_time(ends[0])
     rotated = int(rot.item())
     pairs = launches * (n // 2)
     algo_bytes = 8 * (pairs * 2 * m + rotated * (2 * m + 4 * n)) / launches
@@ -104,6 +158,19 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
     return {"shape": f"{m}x{n} column-graded", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "avg_launch_us": avg_us, "algorithmic_bytes_per_launch": algo_bytes,
             "rotated_pairs": rotated, "pairs": pairs}
+
+
+def profiled_traffic(key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (profiles/traffic.md: lines
+    `traffic <key> <bytes>`, FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().strip("`").split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == key:
+            return float(parts[2])
+    return None
 
 
 def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
@@ -186,6 +253,9 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernels plus a 256 MiB device copy (the counter calibration of the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; multi-GPU runs are not supported",
@@ -201,6 +271,19 @@ def main() -> None:
     cfg = jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=max_sweeps)
     torch.cuda.init()
 
+    if args.roofline_only:
+        eig = eigen_kernel_roofline(torch, jsvd)
+        avg_us, bytes_per_launch, _, _ = round_kernel_roofline(torch, jsvd, a_host, eps)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"eigen_avg_us": eig["avg_launch_us"], "round_avg_us": avg_us,
+                          "round_algorithmic_bytes": bytes_per_launch, "copy_bytes": 2 * src.numel() * 8}),
+              flush=True)
+        return
+
     log(f"[bench] mixed path: warmup {args.warmup}, {args.steps} timed steps")
     mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, args.steps, args.warmup)
     log(f"[bench] mixed: {mixed['solves_per_s']:.2f} solves/s info={mixed['info']} acc={mixed['accuracy']}")
@@ -213,17 +296,20 @@ def main() -> None:
         d = np.abs(np.sort(mixed["sigma"]) - np.sort(plain["sigma"]))
         mixed["accuracy"]["sigma_vs_plain_normwise"] = float(d.max() / np.max(plain["sigma"]))
 
-    log("[bench] round-kernel roofline")
+    log("[bench] dominant kernel of the mixed path (block eigen, FP32) roofline")
+    roofline = eigen_kernel_roofline(torch, jsvd)
+    roofline["traffic"] = profiled_traffic("block_eigen_f32")
+    log("[bench] plain round-kernel roofline")
     avg_us, bytes_per_launch, rotated, pairs = round_kernel_roofline(torch, jsvd, a_host, eps)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
-    roofline = {
+    roofline_plain = {
         "kernel": "round_kernel (plain FP64 fused Gram + rotation + Givens apply, one launch per round)",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": profiled_traffic("round_kernel_tall"),
         "avg_launch_us": avg_us,
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "rotated_pairs": rotated,
@@ -263,7 +349,8 @@ def main() -> None:
         "plain_f64": plain,
         "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
         "roofline": roofline,
-        "roofline_hbm_resident": roofline_hbm,
+        "roofline_plain": roofline_plain,
+        "roofline_plain_hbm_resident": roofline_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -279,6 +279,18 @@ int jsvd_server_get_stats(jsvd_server *server, jsvd_server_stats *stats);
  */
 int jsvd_round_robin_pairs(int64_t n, int32_t round, int32_t *pairs_out);
 
+/*
+ * The inner step of a block round alone (block and mixed paths): for each of `pairs` independent 64 x 64
+ * Gram matrices gram[p] (row-major, device, upper triangle read; float when precision is
+ * JSVD_PRECISION_F32, double when JSVD_PRECISION_F64), one cyclic Jacobi sweep in Gram space in the
+ * round-robin order of 64 columns, rotation parameters in FP64 with the reference's skip test
+ * (src/kernels.cu:503-511 applied to the Gram entries).  q[p] (row-major 64 x 64, same type) receives the
+ * accumulated rotation Q (G <- Q^T G Q), rotations[p] (device int32) the number of rotated column pairs.
+ * No reference counterpart: the reference has no block path.
+ */
+int jsvd_block_eigen(int32_t precision, const void *gram, int64_t pairs, double epsilon, void *q,
+                     int32_t *rotations, void *stream);
+
 /* ---------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, enqueued on `stream`, asynchronous).  They expose the
  * four kernels of the plain path for parity tests and roofline measurement.
@@ -312,8 +324,16 @@ int jsvd_pair_gram_f64(const double *a, int64_t lda, int64_t m, const int32_t *p
  * does nothing (device-side early exit used after convergence).
  */
 int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t round,
-                   double epsilon, int32_t *rot_count, uint64_t *offmax, const int32_t *skip_if_zero,
-                   void *stream);
+                   double epsilon, int32_t *rot_count, uint64_t *offmax, const int32_t *skip_if_zero, void *stream);
+
+/*
+ * `rounds` consecutive fused rounds (rounds first_round, first_round + 1, ... modulo the n - 1 (even n) or n
+ * (odd n) rounds of a sweep), launched back to back on `stream` with the arguments of jsvd_round_f64; rounds =
+ * n - 1 from round 0 is one sweep of the round loop of src/kernels.cu:893-966.
+ */
+int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t first_round,
+                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax,
+                    const int32_t *skip_if_zero, void *stream);
 
 /*
  * Rotation parameters and skip test for `count` pairs: rotate[k] = 1 and (c[k], s[k]) from alpha, beta,

@@ -577,6 +577,32 @@ int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg
     });
 }
 
+int jsvd_block_eigen(int32_t precision, const void *gram, int64_t pairs, double epsilon, void *q,
+                     int32_t *rotations, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(precision == JSVD_PRECISION_F32 || precision == JSVD_PRECISION_F64, "unknown precision.");
+        require(pairs >= 0 && pairs <= (int64_t{1} << 20), "pairs out of range.");
+        require(epsilon > 0.0, "epsilon must be positive.");
+        if (pairs == 0)
+        {
+            return;
+        }
+        require(gram && q && rotations, "buffers must not be null.");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (precision == JSVD_PRECISION_F32)
+        {
+            jsvd::launch_block_eigen(static_cast<const float *>(gram), static_cast<int>(pairs), epsilon,
+                                     static_cast<float *>(q), rotations, s);
+        }
+        else
+        {
+            jsvd::launch_block_eigen(static_cast<const double *>(gram), static_cast<int>(pairs), epsilon,
+                                     static_cast<double *>(q), rotations, s);
+        }
+    });
+}
+
 int jsvd_round_robin_pairs(int64_t n, int32_t round, int32_t *pairs_out)
 {
     int count = 0;
@@ -640,6 +666,34 @@ int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, in
         jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
                            reinterpret_cast<unsigned long long *>(offmax), skip_if_zero,
                            static_cast<hipStream_t>(stream));
+    });
+}
+
+int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t first_round,
+                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax,
+                    const int32_t *skip_if_zero, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(m > 0 && n > 0 && m >= n, "invalid shape (need m >= n > 0).");
+        require(lda >= m && lda % 2 == 0, "lda must be even and >= m.");
+        require(ldv >= n && ldv % 2 == 0, "ldv must be even and >= n.");
+        require(a && v && aligned16(a) && aligned16(v), "a and v must be non-null 16-byte aligned device pointers.");
+        const int64_t N = n + (n & 1);
+        require(n < 2 || (first_round >= 0 && first_round < N - 1), "first_round out of range.");
+        require(rounds >= 0, "rounds must be non-negative.");
+        require(epsilon > 0.0, "epsilon must be positive.");
+        if (n < 2)
+        {
+            return;
+        }
+        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n, false);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        for (int32_t k = 0; k < rounds; ++k)
+        {
+            const int round = static_cast<int>((first_round + k) % (N - 1));
+            jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
+                               reinterpret_cast<unsigned long long *>(offmax), skip_if_zero, s);
+        }
     });
 }
 

@@ -684,6 +684,20 @@ namespace jsvd
                                              double *, int *, int *, unsigned long long *, const int *, double *,
                                              const int *, hipStream_t);
 
+    template <typename T>
+    void launch_block_eigen(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
+    {
+        // A circle of 2 * pairs blocks at round 0 maps slot p to a valid block pair, so the kernel's pair p reads
+        // gram[p] (one partial) and writes q_out[p] and rotations[p]; no column map, norms or sweep counters.
+        const int Nb = 2 * pairs;
+        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream, gram, 1, Nb, Nb, 0, eps,
+                           q_out, rotations, nullptr, nullptr, nullptr, nullptr, nullptr);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    template void launch_block_eigen<float>(const float *, int, double, float *, int *, hipStream_t);
+    template void launch_block_eigen<double>(const double *, int, double, double *, int *, hipStream_t);
+
     void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
                              hipStream_t stream)
     {

@@ -50,6 +50,12 @@ namespace jsvd
                             int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
                             double *norm2, const int *skip_if_zero, hipStream_t stream);
 
+    // The block round's inner step alone, on `pairs` independent kW x kW Gram matrices (gram[p] row-major, upper
+    // triangle read): one cyclic Jacobi sweep in Gram space; q_out[p] (row-major) receives the accumulated
+    // rotation and rotations[p] the number of rotated column pairs.
+    template <typename T>
+    void launch_block_eigen(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream);
+
     // Largest padded column count the one-workgroup ordering kernel sorts (LDS bound).
     constexpr int kSortMax = 8192;
 

@@ -183,6 +183,9 @@ def lib() -> ctypes.CDLL:
                                      _vp, _vp, ctypes.POINTER(SolveInfo)]
         L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
+        L.jsvd_block_eigen.argtypes = [ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
+        L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_double, _vp, _vp, _vp, _vp]
         for name in declared_symbols():
             if name not in ("jsvd_config_default", "jsvd_last_error", "jsvd_version", "jsvd_mixed_config_default"):
                 getattr(L, name).restype = ctypes.c_int
@@ -359,6 +362,16 @@ def round_(a_col, lda: int, m: int, v_col, ldv: int, n: int, rnd: int, eps: floa
                                _stream_handle(stream)))
 
 
+def rounds(a_col, lda: int, m: int, v_col, ldv: int, n: int, first_round: int, count: int, eps: float,
+           rot_count=None, offmax=None, skip_if_zero=None, stream=None) -> None:
+    """`count` consecutive fused rounds from `first_round`, launched back to back (jsvd_rounds_f64)."""
+    check(lib().jsvd_rounds_f64(_ptr(a_col), lda, m, _ptr(v_col), ldv, n, first_round, count, eps,
+                                _ptr(rot_count) if rot_count is not None else None,
+                                _ptr(offmax) if offmax is not None else None,
+                                _ptr(skip_if_zero) if skip_if_zero is not None else None,
+                                _stream_handle(stream)))
+
+
 def rotation_params(alpha, beta, gamma, eps: float, c, s, rotate, stream=None) -> None:
     check(lib().jsvd_rotation_params_f64(_ptr(alpha), _ptr(beta), _ptr(gamma), alpha.numel(), eps, _ptr(c), _ptr(s),
                                          _ptr(rotate), _stream_handle(stream)))
@@ -370,6 +383,23 @@ def offnorm(a_col, lda: int, m: int, n: int, out, stream=None) -> None:
 
 def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) -> None:
     check(lib().jsvd_u_sigma_f64(_ptr(a_col), lda, m, n, eps, _ptr(u), _ptr(sigma), _stream_handle(stream)))
+
+
+def block_eigen(gram, eps: float, q, rotations, stream=None) -> None:
+    """One Gram-space inner sweep on each 64 x 64 matrix of gram (pairs x 64 x 64, float32 or float64 CUDA
+    tensor): q receives the accumulated rotations, rotations (int32, pairs) the rotated-pair counts."""
+    import torch
+    if gram.dim() != 3 or tuple(gram.shape[1:]) != (64, 64) or not gram.is_contiguous():
+        raise ValueError("gram must be a contiguous pairs x 64 x 64 tensor")
+    if q.shape != gram.shape or q.dtype != gram.dtype or not q.is_contiguous():
+        raise ValueError("q must match gram")
+    if rotations.dtype != torch.int32 or rotations.numel() != gram.shape[0]:
+        raise ValueError("rotations must be int32 with one entry per pair")
+    precision = PRECISION_F32 if gram.dtype == torch.float32 else PRECISION_F64
+    if gram.dtype not in (torch.float32, torch.float64):
+        raise ValueError("gram must be float32 or float64")
+    check(lib().jsvd_block_eigen(precision, _ptr(gram), gram.shape[0], eps, _ptr(q), _ptr(rotations),
+                                 _stream_handle(stream)))
 
 
 # ------------------------------------------------------------------------------------------------

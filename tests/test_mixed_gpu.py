@@ -109,3 +109,47 @@ def test_mixed_plan_reuse_deterministic(torch_cuda):
 def test_block_paths_reject_small_n():
     with pytest.raises(ValueError, match="two 32-column blocks"):
         jsvd.solve(gaussian(40, 32, 0), path=jsvd.PATH_MIXED)
+
+
+@pytest.mark.parametrize("dtype,tol", [("float64", 1e-12), ("float32", 2e-4)])
+def test_block_eigen_matches_gram_space_sweep(torch_cuda, dtype, tol):
+    """The inner step of a block round (jsvd_block_eigen) against the numpy restatement of one Gram-space
+    sweep (tests/helpers.py): same rotation order and formula, so Q agrees to rounding (FP64) or to FP32
+    accumulation (FP32); Q is orthogonal and Q^T G Q equals the swept G."""
+    import torch
+    from helpers import gram_space_sweep
+    rng = np.random.default_rng(11)
+    pairs = 5
+    grams = []
+    for p in range(pairs):
+        x = rng.standard_normal((300, 64)) * 10.0 ** (-(p + 1) * np.arange(64) / 63.0)
+        x[:, 63 - p] = 0.0  # a zero column never rotates
+        grams.append(x.T @ x)
+    g = np.stack(grams)
+    tdt = getattr(torch, dtype)
+    d_g = torch.from_numpy(g).to(device="cuda", dtype=tdt).contiguous()
+    d_q = torch.empty_like(d_g)
+    d_rot = torch.zeros(pairs, dtype=torch.int32, device="cuda")
+    eps = 1e-15 if dtype == "float64" else 1e-7
+    jsvd.block_eigen(d_g, eps, d_q, d_rot)
+    torch.cuda.synchronize()
+    q = d_q.cpu().numpy().astype(np.float64)
+    rot = d_rot.cpu().numpy()
+    for p in range(pairs):
+        g_in = d_g[p].cpu().numpy().astype(np.float64)
+        q_ref, rot_ref, g_ref = gram_space_sweep(g_in, eps)
+        assert np.max(np.abs(q[p] - q_ref)) < tol, p
+        if dtype == "float64":
+            assert rot[p] == rot_ref
+        assert rho_orth(q[p]) < (1e-13 if dtype == "float64" else 1e-5)
+        swept = q[p].T @ g_in @ q[p]
+        assert np.max(np.abs(swept - g_ref)) / np.max(np.abs(g_in)) < (1e-13 if dtype == "float64" else 1e-5)
+
+
+def test_block_eigen_rejects_bad_arguments(torch_cuda):
+    import torch
+    g = torch.zeros((2, 64, 64), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        jsvd.block_eigen(g[:, :32, :], 1e-15, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))
+    with pytest.raises(ValueError):
+        jsvd.block_eigen(g, -1.0, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))

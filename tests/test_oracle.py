@@ -215,3 +215,27 @@ def test_pair_stats_brute_force():
         x, y, z = oracle.pair_stats(a, p, q)
         assert abs(x - a[:, p] @ a[:, p]) < 1e-12 and abs(y - a[:, q] @ a[:, q]) < 1e-12
         assert abs(z - a[:, p] @ a[:, q]) < 1e-12
+
+
+def test_gram_space_sweep_reference():
+    """The numpy Gram-space sweep that the block inner step is checked against: each sweep is an orthogonal
+    similarity (Q^T G Q equals the swept G, Q^T Q = I), a few sweeps diagonalise G to its eigenvalues
+    (numpy.linalg.eigvalsh), and a planted sign flip in the rotation breaks the annihilation."""
+    from helpers import gram_space_sweep, block_rotation
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((120, 16)) * 10.0 ** (-2 * np.arange(16) / 15)
+    g = x.T @ x
+    q, rot, g1 = gram_space_sweep(g, 1e-15)
+    assert rot == 16 * 15 // 2
+    assert np.linalg.norm(q.T @ q - np.eye(16)) < 1e-14
+    assert np.max(np.abs(q.T @ g @ q - g1)) < 1e-14 * np.max(np.abs(g))
+    for _ in range(6):
+        _, rot, g1 = gram_space_sweep(g1, 1e-15)
+    assert rot == 0
+    ev = np.linalg.eigvalsh(g)
+    assert np.max(np.abs(np.sort(np.diag(g1)) - ev) / ev) < 1e-12
+    # the rotation annihilates its 2x2 off-diagonal entry; a flipped sine does not
+    app, aqq, apq = 3.0, 1.0, 0.7
+    c, s, _ = block_rotation(app, aqq, apq, 1e-15)
+    assert abs(apq * (c * c - s * s) + (app - aqq) * c * s) < 1e-15
+    assert abs(apq * (c * c - s * s) + (app - aqq) * c * (-s)) > 0.1

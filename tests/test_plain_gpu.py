@@ -142,6 +142,33 @@ def test_one_round_matches_oracle(torch_cuda, m, n):
         assert np.all(d_a.cpu().numpy()[:, m:] == 0.0)
 
 
+@pytest.mark.parametrize("m,n,first,count", [(300, 40, 5, 60), (97, 9, 0, 9)])
+def test_rounds_equal_single_rounds(torch_cuda, m, n, first, count):
+    """jsvd_rounds_f64 (consecutive rounds, wrapping into the next sweep) equals the same rounds launched one
+    by one, bit for bit, rotation counts included."""
+    torch = torch_cuda
+    a = gaussian(m, n, 3)
+    lda, ldv = m + (m % 2), n + (n % 2)
+    per_round = n - 1 + (n % 2)
+    results = []
+    for batched in (False, True):
+        d_a = _dev(torch, _col_padded(a, lda))
+        d_v = _dev(torch, _col_padded(np.eye(n), ldv))
+        rot = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if batched:
+            jsvd.rounds(d_a, lda, m, d_v, ldv, n, first, count, 1e-15, rot_count=rot)
+        else:
+            for k in range(count):
+                jsvd.round_(d_a, lda, m, d_v, ldv, n, (first + k) % per_round, 1e-15, rot_count=rot)
+        torch.cuda.synchronize()
+        results.append((d_a.cpu().numpy(), d_v.cpu().numpy(), int(rot.item())))
+    assert np.array_equal(results[0][0], results[1][0])
+    assert np.array_equal(results[0][1], results[1][1])
+    assert results[0][2] == results[1][2] > 0
+    with pytest.raises(ValueError):
+        jsvd.rounds(d_a, lda, m, d_v, ldv, n, per_round, 1, 1e-15)
+
+
 def test_round_skip_flag(torch_cuda):
     torch = torch_cuda
     a = gaussian(64, 8, 1)
