@@ -160,6 +160,35 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
             "rotated_pairs": rotated, "pairs": pairs}
 
 
+def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=4):
+    """The tall workload as a *.mat file of `records` copies, solved by the streamed file path (mmap -> pinned
+    double buffer -> HBM on a copy stream -> device decode, overlapping the next record's load with the current
+    solve) on the mixed path, U / sigma / V written back to a *.mat file.  PCIe- and file-inclusive."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="jsvd_bench_")
+    try:
+        src, dst = os.path.join(d, "tall.mat"), os.path.join(d, "tall_out.mat")
+        jsvd.write_mat(src, [a_host] * records)
+        rep = jsvd.mat_stream_solve(src, dst, cfg, path=jsvd.PATH_MIXED)
+        out = jsvd.read_mat(dst)
+        same = all(np.array_equal(out[3 * k + 1][0], sigma_mixed) for k in range(records))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    r = rep.as_dict()
+    return {
+        "records": r["records"],
+        "solves_per_s": r["records"] / r["seconds_total"],
+        "seconds_total": r["seconds_total"],
+        "seconds_load": r["seconds_load"],
+        "seconds_solve": r["seconds_solve"],
+        "seconds_output": r["seconds_output"],
+        "load_hidden": r["seconds_load"] <= r["seconds_solve"],
+        "sigma_equals_resident_solve": bool(same),
+        "note": "file in / file out; the load of record k+1 runs on a copy stream while record k is solved",
+    }
+
+
 def profiled_traffic(key):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (profiles/traffic.md: lines
     `traffic <key> <bytes>`, FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), or None."""
@@ -253,6 +282,10 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
+    ap.add_argument("--no-file", action="store_true", help="skip the streamed *.mat file leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving leg")
+    ap.add_argument("--serving-rate", type=float, default=1000.0)
+    ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernels plus a 256 MiB device copy (the counter calibration of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
@@ -295,6 +328,27 @@ def main() -> None:
         # sigma agreement of the two GPU paths (sorted)
         d = np.abs(np.sort(mixed["sigma"]) - np.sort(plain["sigma"]))
         mixed["accuracy"]["sigma_vs_plain_normwise"] = float(d.max() / np.max(plain["sigma"]))
+
+    file_stream = None
+    if args.workload == "tall" and not args.no_file:
+        log("[bench] streamed *.mat file path (mixed)")
+        file_stream = file_stream_leg(jsvd, a_host, cfg, mixed["sigma"])
+        log(f"[bench] file stream: {file_stream}")
+
+    serving = None
+    if not args.no_serving:
+        from jsvd import serving as serving_mod
+        log(f"[bench] serving: Poisson arrivals at {args.serving_rate}/s for {args.serving_seconds}s")
+        pool = serving_mod.request_pool(48)
+        srv = jsvd.Server(lanes=4, large_path=jsvd.PATH_MIXED)
+        try:
+            serving = serving_mod.run_poisson(srv, pool, args.serving_rate, args.serving_seconds, check_every=10)
+        finally:
+            srv.close()
+        serving["workload"] = ("serving mix: m = exp U(ln16, ln2048), n ~ U{4..min(m,512)}, kappa 10^U(1,10), "
+                               "relerr 1e-6 / 1e-12 alternating, 48 distinct matrices cycled, 4 lanes, HBM "
+                               "budget 64 GiB, large requests on the mixed path")
+        log(f"[bench] serving: {serving}")
 
     log("[bench] dominant kernel of the mixed path (block eigen, FP32) roofline")
     roofline = eigen_kernel_roofline(torch, jsvd)
@@ -348,6 +402,8 @@ def main() -> None:
         "mixed": mixed,
         "plain_f64": plain,
         "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
+        "file_stream_mixed": file_stream,
+        "serving": serving,
         "roofline": roofline,
         "roofline_plain": roofline_plain,
         "roofline_plain_hbm_resident": roofline_hbm,

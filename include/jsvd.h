@@ -271,6 +271,36 @@ int jsvd_server_wait(jsvd_server *server, uint64_t id, double *u, double *sigma,
 
 int jsvd_server_get_stats(jsvd_server *server, jsvd_server_stats *stats);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Streamed *.mat solve (file in, file out) on one GPU.
+ * ------------------------------------------------------------------------------------------------- */
+
+typedef struct jsvd_mat_stream_report
+{
+    int64_t records;          /* matrices solved */
+    int64_t payload_bytes;    /* input payload bytes moved to HBM */
+    double seconds_total;     /* open to last result written */
+    double seconds_load;      /* loader thread: mmap -> pinned copy, H2D copy and device decode */
+    double seconds_solve;     /* solver thread: solves (the load of the next record overlaps these) */
+    double seconds_output;    /* solver thread: device encode, copy back, append to the output file */
+    int32_t sweeps_fp32_last; /* sweeps of the last record */
+    int32_t sweeps_fp64_last;
+} jsvd_mat_stream_report;
+
+/*
+ * Solves every record of the *.mat file input_path (u64_be rows | u64_be cols | rows*cols f64_be, row-major,
+ * to end of file) and, when output_path is not NULL, writes U (m x n), sigma (1 x n) and V (n x n) per record,
+ * in input order, as *.mat records: the reference pipeline's *.mat-to-*.mat run (src/pipeline.cpp:754-863 with
+ * the record reader of src/io.cu:1101-1168 and the packets of src/pipeline.cpp:711-739).  The file is
+ * memory-mapped; a loader thread copies record k+1 into a pinned staging buffer (two, alternating), sends it to
+ * HBM on a copy stream and decodes it there (big-endian to double) while record k is solved on a compute
+ * stream.  config / mixed select the solver (NULL: defaults, plain FP64); records narrower than 33 columns take
+ * the plain path.  Every record's shape is validated before the first solve (m >= n > 0, else
+ * JSVD_ERR_INVALID_ARGUMENT); truncated files fail with the reader's messages.
+ */
+int jsvd_mat_stream_solve(const char *input_path, const char *output_path, const jsvd_config *config,
+                          const jsvd_mixed_config *mixed, jsvd_mat_stream_report *report);
+
 /*
  * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
  * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the

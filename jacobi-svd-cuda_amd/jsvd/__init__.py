@@ -613,3 +613,66 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------------------------------------
+# *.mat files and the streamed file solve (jsvd_mat_stream_solve)
+# ------------------------------------------------------------------------------------------------
+def write_mat(path: str, matrices) -> None:
+    """Writes row-major float64 matrices as *.mat records (u64_be rows | u64_be cols | f64_be payload)."""
+    with open(path, "wb") as f:
+        for m in matrices:
+            arr = np.ascontiguousarray(np.asarray(m, dtype=np.float64))
+            if arr.ndim != 2:
+                raise ValueError("matrices must be 2-D")
+            f.write(np.array(arr.shape, dtype=">u8").tobytes())
+            f.write(arr.astype(">f8").tobytes())
+
+
+def read_mat(path: str) -> list:
+    """Reads every *.mat record of a file (memory-mapped) into float64 arrays."""
+    buf = np.memmap(path, dtype=np.uint8, mode="r") if os.path.getsize(path) else np.zeros(0, np.uint8)
+    out, pos = [], 0
+    while pos < buf.size:
+        if buf.size - pos < 16:
+            raise RuntimeError("Truncated *.mat header.")
+        rows, cols = (int(x) for x in np.frombuffer(buf[pos:pos + 16].tobytes(), dtype=">u8"))
+        pos += 16
+        nbytes = rows * cols * 8
+        if buf.size - pos < nbytes:
+            raise RuntimeError("Truncated *.mat payload.")
+        out.append(np.frombuffer(buf[pos:pos + nbytes].tobytes(), dtype=">f8").astype(np.float64).reshape(rows, cols))
+        pos += nbytes
+    return out
+
+
+class MatStreamReport(ctypes.Structure):
+    """jsvd_mat_stream_report (include/jsvd.h)."""
+    _fields_ = [
+        ("records", ctypes.c_int64),
+        ("payload_bytes", ctypes.c_int64),
+        ("seconds_total", ctypes.c_double),
+        ("seconds_load", ctypes.c_double),
+        ("seconds_solve", ctypes.c_double),
+        ("seconds_output", ctypes.c_double),
+        ("sweeps_fp32_last", ctypes.c_int32),
+        ("sweeps_fp64_last", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def mat_stream_solve(input_path: str, output_path: Optional[str], config: Optional[JacobiSvdConfig] = None,
+                     path: int = PATH_PLAIN_F64, mixed: Optional[MixedConfig] = None) -> MatStreamReport:
+    """Solves every record of a *.mat file, overlapping the load of the next record with the current solve;
+    writes U, sigma (1 x n), V per record to output_path (None: results are discarded)."""
+    L = lib()
+    L.jsvd_mat_stream_solve.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(JsvdConfig),
+                                        ctypes.POINTER(MixedConfig), ctypes.POINTER(MatStreamReport)]
+    rep = MatStreamReport()
+    cfg = _as_c_config(config)
+    mx = mixed if mixed is not None else mixed_config(path)
+    check(L.jsvd_mat_stream_solve(input_path.encode(), output_path.encode() if output_path else None,
+                                  ctypes.byref(cfg), ctypes.byref(mx), ctypes.byref(rep)))
+    return rep

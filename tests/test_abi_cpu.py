@@ -1,6 +1,8 @@
 """Host-side checks of libjsvd.so that need no GPU: it loads, exports every symbol include/jsvd.h
 declares, carries the reference defaults, validates inputs like the reference (src/kernels.cu:807-833,
 338-356) before touching the device, and its closed-form schedule equals the oracle's schedule."""
+import os
+
 import numpy as np
 import pytest
 
@@ -56,3 +58,17 @@ def test_closed_form_schedule_equals_oracle(n):
 def test_schedule_round_out_of_range():
     with pytest.raises(ValueError):
         jsvd.round_robin_pairs(8, 7)
+
+
+def test_mat_helpers_match_hand_computed_record(tmp_path):
+    """jsvd.write_mat / read_mat against the hand-computed bytes of a 1 x 2 record [1.0, -2.0]."""
+    import jsvd
+    golden = open(os.path.join(os.path.dirname(__file__), "golden", "mat_1x2.bin"), "rb").read()
+    p = str(tmp_path / "x.mat")
+    jsvd.write_mat(p, [np.array([[1.0, -2.0]])])
+    assert open(p, "rb").read() == golden
+    back = jsvd.read_mat(p)
+    assert len(back) == 1 and back[0].shape == (1, 2) and np.array_equal(back[0], [[1.0, -2.0]])
+    open(p, "wb").write(golden[:-3])
+    with pytest.raises(RuntimeError, match="payload"):
+        jsvd.read_mat(p)

@@ -75,3 +75,21 @@ def test_server_budget_admission(torch_cuda):
     srv.close()
     assert all(r[3].status == 0 for r in res)
     assert st["budget_peak"] <= budget and st["budget_in_use"] == 0
+
+
+def test_poisson_serving_short_run(torch_cuda):
+    """A short open-loop run of the serving harness (Poisson arrivals, serving-workload shapes up to 512 x 128):
+    every request completes, latency percentiles are reported, and sigma meets the requested accuracy
+    (normwise vs the construction's exact singular values: 1e-5 for relerr 1e-6, 1e-12 for relerr 1e-12)."""
+    from jsvd import serving
+    pool = serving.request_pool(24, seed=9, max_m=512, max_n=128)
+    srv = jsvd.Server(lanes=4, large_path=jsvd.PATH_MIXED)
+    try:
+        rep = serving.run_poisson(srv, pool, rate=200.0, duration=1.0, check_every=1)
+    finally:
+        srv.close()
+    assert rep["failed"] == 0 and rep["completed"] == rep["submitted"] > 50
+    assert rep["p50_ms"] <= rep["p99_ms"]
+    assert rep["max_sigma_err_fp32_requests"] < 1e-5
+    assert rep["max_sigma_err_fp64_requests"] < 1e-12
+    assert rep["server"]["max_in_flight"] > 1
