@@ -146,6 +146,15 @@ int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd
 /* Report of the last solve on this plan. */
 int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info);
 
+/*
+ * Re-targets a plan at an m x n problem (m >= n) and epsilon without reallocating: the shape must fit the
+ * buffers allocated for the shape the plan was created with (m <= created m, n <= created n; block paths also
+ * need n >= 33), else JSVD_ERR_INVALID_ARGUMENT and the plan is unchanged.  Lets one plan per stream serve
+ * requests of many shapes (the request loop keeps one such plan per lane).  No reference counterpart: the
+ * reference allocates per call (src/kernels.cu:844-885).
+ */
+int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon);
+
 /* One-shot host-pointer solve through the path selected by `mixed` (NULL: plain FP64). */
 int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
                    double *u, double *sigma, double *v, jsvd_solve_info *info);

@@ -110,6 +110,8 @@ struct jsvd_plan
     int64_t n = 0;
     int64_t lda = 0; // even, >= m
     int64_t ldv = 0; // even, >= n
+    int64_t cap_m = 0; // shape the buffers were allocated for (reshape targets fit inside it)
+    int64_t cap_n = 0;
     jsvd_config cfg{};
     jsvd::RoundShape shape{};
     hipStream_t stream = nullptr;
@@ -164,6 +166,8 @@ struct jsvd_plan
     {
         m = rows;
         n = cols;
+        cap_m = rows;
+        cap_n = cols;
         cfg = config;
         path = mixed.path;
         if (user_stream)
@@ -182,6 +186,8 @@ struct jsvd_plan
     {
         m = rows;
         n = cols;
+        cap_m = rows;
+        cap_n = cols;
         cfg = config;
         lda = jsvd::round_up(m, 2);
         ldv = jsvd::round_up(n, 2);
@@ -207,6 +213,37 @@ struct jsvd_plan
         // Padding rows m..lda-1 stay zero for the plan's lifetime: transposes only write rows < m and the
         // round kernel reads/writes the padding row of an odd m as zero.
         JSVD_HIP_CHECK(hipMemsetAsync(a_col, 0, sizeof(double) * lda * n, stream));
+    }
+
+    // Re-targets the plan at rows x cols with epsilon `eps` without reallocating; false (plan unchanged) when the
+    // shape does not fit the buffers allocated at creation.
+    bool reshape(int64_t rows, int64_t cols, double eps)
+    {
+        if (rows < 1 || cols < 1 || rows < cols || rows > cap_m || cols > cap_n)
+        {
+            return false;
+        }
+        if (block)
+        {
+            if (!block->reshape(rows, cols))
+            {
+                return false;
+            }
+            block->set_epsilon(eps);
+        }
+        else if (rows != m || cols != n)
+        {
+            const int64_t new_lda = jsvd::round_up(rows, 2);
+            // padding rows must read as zero: clear the new working extent
+            JSVD_HIP_CHECK(hipMemsetAsync(a_col, 0, sizeof(double) * new_lda * cols, stream));
+            lda = new_lda;
+            ldv = jsvd::round_up(cols, 2);
+            shape = jsvd::choose_round_shape(rows, cols, jsvd::use_wide_shape(rows, cols, cfg));
+        }
+        m = rows;
+        n = cols;
+        cfg.epsilon = eps;
+        return true;
     }
 
     // Device-resident solve: transpose in, sweep loop, sigma/U extraction, V transpose out.
@@ -311,10 +348,11 @@ struct jsvd_plan
     {
         if (!d_a)
         {
-            JSVD_HIP_CHECK(hipMalloc(&d_a, sizeof(double) * m * n));
-            JSVD_HIP_CHECK(hipMalloc(&d_u, sizeof(double) * m * n));
-            JSVD_HIP_CHECK(hipMalloc(&d_sigma, sizeof(double) * n));
-            JSVD_HIP_CHECK(hipMalloc(&d_v, sizeof(double) * n * n));
+            // sized for the capacity shape: a reshaped plan reuses them
+            JSVD_HIP_CHECK(hipMalloc(&d_a, sizeof(double) * cap_m * cap_n));
+            JSVD_HIP_CHECK(hipMalloc(&d_u, sizeof(double) * cap_m * cap_n));
+            JSVD_HIP_CHECK(hipMalloc(&d_sigma, sizeof(double) * cap_n));
+            JSVD_HIP_CHECK(hipMalloc(&d_v, sizeof(double) * cap_n * cap_n));
         }
     }
 
@@ -551,6 +589,16 @@ int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd
         require(plan != nullptr, "plan output pointer must not be null.");
         *plan = nullptr;
         *plan = make_plan(m, n, cfg, mixed, static_cast<hipStream_t>(stream)).release();
+    });
+}
+
+int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan must not be null.");
+        require(epsilon > 0.0, "epsilon must be positive.");
+        jsvd::validate_shape(m, n);
+        require(plan->reshape(m, n, epsilon), "shape does not fit the plan's capacity.");
     });
 }
 

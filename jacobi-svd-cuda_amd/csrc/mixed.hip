@@ -45,6 +45,7 @@ namespace jsvd
         {
             throw_invalid("block and mixed paths need at least two 32-column blocks (n >= 33).");
         }
+        fp32_tol_requested_ = mixed_.fp32_tolerance;
         if (mixed_.fp32_tolerance <= 0.0)
         {
             mixed_.fp32_tolerance = default_fp32_tolerance(m);
@@ -53,6 +54,11 @@ namespace jsvd
         const size_t a_elems = static_cast<size_t>(g_.lda) * g_.n_pad;
         const size_t v_elems = static_cast<size_t>(g_.ldv) * g_.n_pad;
         const size_t part_elems = static_cast<size_t>(g_.pairs()) * g_.slabs_a * kW * kW;
+        cap_a_elems_ = a_elems;
+        cap_v_elems_ = v_elems;
+        cap_part_elems_ = part_elems;
+        cap_pairs_ = g_.pairs();
+        cap_npad_ = g_.n_pad;
         JSVD_HIP_CHECK(hipMalloc(&a64_, sizeof(double) * a_elems));
         JSVD_HIP_CHECK(hipMalloc(&v64_, sizeof(double) * v_elems));
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
@@ -191,6 +197,31 @@ namespace jsvd
         {
             JSVD_HIP_CHECK(hipMemcpyAsync(v, cur, sizeof(double) * g_.ldv * g_.n_pad, hipMemcpyDeviceToDevice, stream_));
         }
+    }
+
+    bool BlockSolver::reshape(int64_t m, int64_t n)
+    {
+        const BlockGeometry g = BlockGeometry::make(m, n);
+        const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
+        const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
+        const size_t part_elems = static_cast<size_t>(g.pairs()) * g.slabs_a * kW * kW;
+        if (g.nb < 2 || a_elems > cap_a_elems_ || v_elems > cap_v_elems_ || part_elems > cap_part_elems_ ||
+            g.pairs() > cap_pairs_ || g.n_pad > cap_npad_)
+        {
+            return false;
+        }
+        if (g.m != g_.m || g.n != g_.n)
+        {
+            // padding rows and columns of the new extent must read as zero
+            JSVD_HIP_CHECK(hipMemsetAsync(a64_, 0, sizeof(double) * a_elems, stream_));
+            if (a64b_)
+            {
+                JSVD_HIP_CHECK(hipMemsetAsync(a64b_, 0, sizeof(double) * a_elems, stream_));
+            }
+        }
+        g_ = g;
+        mixed_.fp32_tolerance = (fp32_tol_requested_ > 0.0) ? fp32_tol_requested_ : default_fp32_tolerance(m);
+        return true;
     }
 
     void BlockSolver::solve_device(const double *d_a, double *d_u, double *d_sigma, double *d_v,

@@ -23,6 +23,11 @@ namespace jsvd
         // d_a row-major m x n (not modified); outputs row-major U (m x n), sigma (n), V (n x n).
         void solve_device(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
 
+        // Re-targets the solver at an m x n problem inside the buffers allocated at construction (the capacity
+        // shape); false (nothing changed) when it does not fit or n < 33.
+        bool reshape(int64_t m, int64_t n);
+        void set_epsilon(double eps) { cfg_.epsilon = eps; }
+
     private:
         template <typename T>
         int run_sweeps(T *a, T *v, double eps, int max_sweeps, int *rot, unsigned long long *off, double &last_off);
@@ -49,6 +54,13 @@ namespace jsvd
         unsigned long long *off_ = nullptr;  // [2 * cap]
         int *h_rot_ = nullptr;               // pinned mirror
         int cap_ = 0;
+        // capacity of the geometry-sized buffers
+        size_t cap_a_elems_ = 0;
+        size_t cap_v_elems_ = 0;
+        size_t cap_part_elems_ = 0;
+        int cap_pairs_ = 0;
+        int64_t cap_npad_ = 0;
+        double fp32_tol_requested_ = 0.0;
         hipEvent_t done_[2] = {nullptr, nullptr};
     };
 } // namespace jsvd

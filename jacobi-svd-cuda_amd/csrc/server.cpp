@@ -180,18 +180,25 @@ struct jsvd_server
     {
         hipStream_t stream = nullptr;
         jsvd_batch *batch = nullptr;
-        jsvd_plan *plan = nullptr;
-        int64_t plan_m = 0;
-        int64_t plan_n = 0;
-        double plan_eps = 0.0;
+        // One plan per path, sized for the largest request the lane has seen and re-targeted at each request's
+        // shape and tolerance (jsvd_plan_reshape): no allocation per request.
+        jsvd_plan *plan_block = nullptr;
+        int64_t block_cap_m = 0;
+        int64_t block_cap_n = 0;
+        jsvd_plan *plan_plain = nullptr;
+        int64_t plain_cap_m = 0;
+        int64_t plain_cap_n = 0;
         PinnedBuffer<double> a, u, s, v;
         PinnedBuffer<jsvd_batch_item> items;
         PinnedBuffer<int32_t> sweeps;
         ~LaneState()
         {
-            if (plan)
+            for (jsvd_plan *p : {plan_block, plan_plain})
             {
-                (void)jsvd_plan_destroy(plan);
+                if (p)
+                {
+                    (void)jsvd_plan_destroy(p);
+                }
             }
             if (batch)
             {
@@ -214,31 +221,35 @@ struct jsvd_server
 
     void run_large(LaneState &ls, Request &r)
     {
-        if (!ls.plan || ls.plan_m != r.m || ls.plan_n != r.n || ls.plan_eps != r.tol)
+        // the block paths need two 32-column blocks
+        const bool block = cfg.large_path != JSVD_PATH_PLAIN_F64 && r.n > 32;
+        jsvd_plan *&plan = block ? ls.plan_block : ls.plan_plain;
+        int64_t &cap_m = block ? ls.block_cap_m : ls.plain_cap_m;
+        int64_t &cap_n = block ? ls.block_cap_n : ls.plain_cap_n;
+        if (!plan || r.m > cap_m || r.n > cap_n)
         {
-            if (ls.plan)
+            if (plan)
             {
-                (void)jsvd_plan_destroy(ls.plan);
-                ls.plan = nullptr;
+                (void)jsvd_plan_destroy(plan);
+                plan = nullptr;
             }
+            cap_m = std::max(cap_m, r.m); // cap_m >= r.m >= r.n and every earlier n: a valid m >= n shape
+            cap_n = std::max(cap_n, r.n);
             jsvd_config c;
             jsvd_config_default(&c);
             c.epsilon = r.tol;
             c.max_sweeps = cfg.max_sweeps;
             jsvd_mixed_config mx;
             jsvd_mixed_config_default(&mx);
-            // the block paths need two 32-column blocks
-            mx.path = (cfg.large_path != JSVD_PATH_PLAIN_F64 && r.n > 32) ? cfg.large_path : JSVD_PATH_PLAIN_F64;
-            raise(jsvd_plan_create_ex(r.m, r.n, &c, &mx, ls.stream, &ls.plan));
-            ls.plan_m = r.m;
-            ls.plan_n = r.n;
-            ls.plan_eps = r.tol;
+            mx.path = block ? cfg.large_path : JSVD_PATH_PLAIN_F64;
+            raise(jsvd_plan_create_ex(cap_m, cap_n, &c, &mx, ls.stream, &plan));
         }
+        raise(jsvd_plan_reshape(plan, r.m, r.n, r.tol));
         r.u.resize(r.m * r.n);
         r.sigma.resize(r.n);
         r.v.resize(r.n * r.n);
         int32_t sweeps = 0;
-        raise(jsvd_plan_solve_host(ls.plan, r.a.data(), r.u.data(), r.sigma.data(), r.v.data(), &sweeps));
+        raise(jsvd_plan_solve_host(plan, r.a.data(), r.u.data(), r.sigma.data(), r.v.data(), &sweeps));
         r.sweeps = sweeps;
         r.batch_size = 1;
     }

@@ -293,6 +293,13 @@ class Plan:
         check(lib().jsvd_plan_last_info(self._h, ctypes.byref(info)))
         return info
 
+    def reshape(self, m: int, n: int, epsilon: float) -> None:
+        """Re-targets the plan at m x n and epsilon inside its allocated capacity (jsvd_plan_reshape)."""
+        L = lib()
+        L.jsvd_plan_reshape.argtypes = [_vp, _i64, _i64, ctypes.c_double]
+        check(L.jsvd_plan_reshape(self._h, int(m), int(n), float(epsilon)))
+        self.m, self.n = int(m), int(n)
+
     def close(self) -> None:
         if self._h:
             check(lib().jsvd_plan_destroy(self._h))

@@ -153,3 +153,34 @@ def test_block_eigen_rejects_bad_arguments(torch_cuda):
         jsvd.block_eigen(g[:, :32, :], 1e-15, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))
     with pytest.raises(ValueError):
         jsvd.block_eigen(g, -1.0, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))
+
+
+@pytest.mark.parametrize("path", [jsvd.PATH_PLAIN_F64, jsvd.PATH_BLOCK_F64, jsvd.PATH_MIXED])
+def test_plan_reshape_equals_fresh_plans(torch_cuda, path):
+    """One plan re-targeted at several smaller shapes and epsilons (jsvd_plan_reshape) gives, bit for bit, the
+    results of fresh plans created for each shape; shapes beyond the capacity are rejected."""
+    import torch
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    big = jsvd.Plan(400, 160, cfg, path=path)
+    shapes = [(400, 160), (257, 97), (300, 40), (101, 101), (399, 159)]
+    if path == jsvd.PATH_PLAIN_F64:
+        shapes.append((50, 7))
+    for k, (m, n) in enumerate(shapes):
+        eps = 1e-14 if k % 2 == 0 else 1e-12
+        a = gaussian(m, n, 40 + k)
+        big.reshape(m, n, eps)
+        d_a = torch.from_numpy(a).cuda()
+        outs = []
+        for plan in (big, jsvd.Plan(m, n, jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=60), path=path)):
+            d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
+            d_s = torch.empty(n, dtype=torch.float64, device="cuda")
+            d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
+            plan.solve_device(d_a, d_u, d_s, d_v)
+            outs.append((d_u.cpu().numpy(), d_s.cpu().numpy(), d_v.cpu().numpy()))
+        for x, y in zip(outs[0], outs[1]):
+            assert np.array_equal(x, y), (m, n)
+    with pytest.raises(ValueError):
+        big.reshape(401, 160, 1e-14)
+    with pytest.raises(ValueError):
+        big.reshape(300, 161, 1e-14)
+    big.close()
