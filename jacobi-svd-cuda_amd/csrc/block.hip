@@ -148,31 +148,72 @@ namespace jsvd
 
         // Reduction of the slab partials and one inner cyclic Jacobi sweep in Gram space.
         //
-        // G is symmetric and kept only in its upper triangle (g[min][max]).  Each inner round pairs the 64
-        // columns by the round-robin circle (pair u of a round = circle positions u and 63 - u).  A round has
-        // two phases separated by a barrier:
-        //   phase 1: lanes 0..31 of wave 0 form the 32 rotations of the round in FP64 (reference skip test) and
-        //            publish them; threads 0..527 load their 2x2 block of G (block (t, u), t <= u, of the 32 x 32
-        //            pair grid); the Q waves apply the previous round's rotations to Q (off the critical path);
-        //   phase 2: threads 0..527 write J_t^T G J_u back in place.
-        // The instruction issue of the 16 waves bounds a round, so the work is packed into as few waves as it
-        // needs: G blocks on waves 0..8, Q on waves 12..15.  Q is kept in registers in circle-position order:
-        // thread (row k, segment s) of the Q waves holds Q[k][col(p)] for the positions p = 8s..8s+7 and their
-        // partners 63-p, so every rotation of a round is local to a thread; after a round the positions move by
-        // one along the circle (1..63 cyclically, 0 fixed), which moves one value to each neighbouring segment.
-        // Column indices advance incrementally (col(p, r+1) = col(p, r) - 1 on 1..63).  A block pair none of
-        // whose 2016 column pairs passes the skip test is left untouched.
+        // G is symmetric and kept only in its upper triangle (g[min][max]), in two LDS buffers.  Each inner round
+        // pairs the 64 columns by the round-robin circle (pair u of a round = circle positions u and 63 - u).
+        // Inner round r is one phase and one barrier:
+        //   * the G waves (one 2x2 block (t, u), t <= u, of the 32 x 32 pair grid per thread) write
+        //     J_t^T G_r J_u into the other buffer, with the rotations of round r published in the previous phase;
+        //   * the Q waves apply the same rotations to Q;
+        //   * wave 0 meanwhile forms the rotations of round r+1 (lane l: pair l % 32, FP64, reference skip test)
+        //     without waiting for G_{r+1}: the three entries a rotation needs are the two new diagonal entries of
+        //     its columns and their new off-diagonal entry, each a 2x2 update of entries of G_r with rotations
+        //     of round r that wave 0 already holds (exchanged between its lanes), evaluated by the same
+        //     expressions as the G waves use.
+        // So the FP64 rotation chain of the next round overlaps the bulk update of this one.  Vector-instruction
+        // issue bounds the bulk update, so it is packed into as few waves as it needs.  Q is kept in registers in
+        // circle-position order: thread (row k, segment s) of the Q waves holds Q[k][col(p)] for the positions
+        // p = 8s..8s+7 and their partners 63-p, so every rotation of a round is local to a thread; after a round
+        // the positions move by one along the circle (1..63 cyclically, 0 fixed), which moves one value to each
+        // neighbouring segment.  Column indices advance incrementally (col(p, r+1) = col(p, r) - 1 on 1..63).  A
+        // block pair none of whose 2016 column pairs passes the skip test is left untouched.
         constexpr int kEigenThreads = 1024;
         constexpr int kPairs = kW / 2;
         constexpr int kTriBlocks = kPairs * (kPairs + 1) / 2;   // 2x2 blocks of the upper triangle (528)
         constexpr int kQThreads = 256;                          // 64 rows x 4 segments of 8 position pairs
-        constexpr int kQFirst = kEigenThreads - kQThreads;      // Q waves are the last four
-        static_assert(kW == 64 && kTriBlocks <= kQFirst, "eigen thread roles assume 64 columns");
+        constexpr int kQFirst = kEigenThreads - kQThreads;      // Q waves are the last four (12..15)
+        // Waves are spread round-robin over the CU's four SIMDs (wave w on SIMD w % 4).  Wave 0 forms the
+        // rotations, so the G blocks go to the waves 1..11 not on its SIMD (1, 2, 3, 5, 6, 7, 9, 10, 11): every
+        // SIMD then issues one Q wave and either the rotation wave or a third G wave.
+        constexpr int kGWaves = 9;
+        static_assert(kW == 64 && kGWaves * 64 >= kTriBlocks, "eigen thread roles assume 64 columns");
+
+        // Index of this thread among the G threads, or -1.
+        __device__ inline int g_thread_index(int tid)
+        {
+            const int w = tid / 64;
+            if (w < 1 || w > 11 || (w % 4) == 0)
+            {
+                return -1;
+            }
+            const int gi = (w - 1 - w / 4) * 64 + tid % 64;
+            return gi < kTriBlocks ? gi : -1;
+        }
 
         // Column at a circle position in the next inner round.
         __device__ inline int next_column(int c)
         {
             return c == 0 ? 0 : (c == 1 ? kW - 1 : c - 1);
+        }
+
+        // Circle position, in the previous round, of the column now at position p.
+        __device__ inline int previous_position(int p)
+        {
+            return p == 0 ? 0 : (p == 1 ? kW - 1 : p - 1);
+        }
+
+        // J_t^T [g11 g12; g21 g22] J_u for the rotations (ct, st) of the row pair and (cu, su) of the column pair
+        // (x' = c x - s y, y' = s x + c y), with explicit fused multiply-adds so that every evaluation of the same
+        // inputs gives the same bits.
+        template <typename T>
+        __device__ inline void rotate_block(T ct, T st, T cu, T su, T g11, T g12, T g21, T g22, T &n11, T &n12,
+                                            T &n21, T &n22)
+        {
+            const T h11 = fma(ct, g11, -(st * g21)), h12 = fma(ct, g12, -(st * g22));
+            const T h21 = fma(st, g11, ct * g21), h22 = fma(st, g12, ct * g22);
+            n11 = fma(cu, h11, -(su * h12));
+            n12 = fma(su, h11, cu * h12);
+            n21 = fma(cu, h21, -(su * h22));
+            n22 = fma(su, h21, cu * h22);
         }
 
         // One inner round on a thread's Q registers: rotate its 8 position pairs, then move the positions one step
@@ -232,8 +273,8 @@ namespace jsvd
             int *__restrict__ apply_flag, int *__restrict__ rot_count, unsigned long long *__restrict__ offmax,
             const int *__restrict__ perm, double *__restrict__ norm2, const int *__restrict__ skip_if_zero)
         {
-            __shared__ T g[kW][kW + 1];
-            __shared__ T cs[2][kPairs][2];              // rotations of the current and the previous round
+            __shared__ T g[2][kW][kW + 1];              // G of the current round and of the next
+            __shared__ T cs[2][kPairs][2];              // rotations of the current round and of the next
             __shared__ unsigned long long rmask[2];     // bit u: pair u rotates
             __shared__ double red[kEigenThreads / 64];
             __shared__ int any;
@@ -251,10 +292,12 @@ namespace jsvd
             }
             const int tid = threadIdx.x;
             // This thread's 2x2 block (t, u), t <= u, of the upper triangle of the 32 x 32 pair grid.
+            const int gi = g_thread_index(tid);
+            const bool gthr = gi >= 0;
             int bt = 0, bu = 0;
-            if (tid < kTriBlocks)
+            if (gthr)
             {
-                int rem = tid;
+                int rem = gi;
                 while (rem >= kPairs - bt)
                 {
                     rem -= kPairs - bt;
@@ -290,7 +333,7 @@ namespace jsvd
                     {
                         acc += base[static_cast<int64_t>(s) * (kW * kW) + e];
                     }
-                    g[i][j] = acc;
+                    g[0][i][j] = acc;
                 }
             }
             if (tid == 0)
@@ -321,9 +364,9 @@ namespace jsvd
                 const int j = e % kW;
                 if (i < j)
                 {
-                    const double gii = static_cast<double>(g[i][i]);
-                    const double gjj = static_cast<double>(g[j][j]);
-                    const double gij = static_cast<double>(g[i][j]);
+                    const double gii = static_cast<double>(g[0][i][i]);
+                    const double gjj = static_cast<double>(g[0][j][j]);
+                    const double gij = static_cast<double>(g[0][i][j]);
                     best = fmax(best, dev::pair_cosine(gii, gjj, gij));
                     const double rhs = eps * sqrt(fmax(gii, 0.0) * fmax(gjj, 0.0));
                     need = need || (fabs(gij) > rhs && fabs(gij) > 1.0e-300);
@@ -340,82 +383,116 @@ namespace jsvd
             }
             __syncthreads();
             int my_rotations = 0;
-            const bool gthr = tid < kTriBlocks;
             int x1 = dev::circle_at_fast(bt, 0, kW), x2 = dev::circle_at_fast(kW - 1 - bt, 0, kW);
             int z1 = dev::circle_at_fast(bu, 0, kW), z2 = dev::circle_at_fast(kW - 1 - bu, 0, kW);
-            const int pu = tid % kPairs; // wave 0: the pair this lane rotates
-            int p1 = dev::circle_at_fast(pu, 0, kW), p2 = dev::circle_at_fast(kW - 1 - pu, 0, kW);
             const bool run = needed != 0;
+            // Wave 0: the rotation of pair pu of the current round and the three G entries it was formed from.
+            const int pu = tid % kPairs;
+            T my_c = 1, my_s = 0, gpp = 0, gqq = 0, gpq = 0;
+            if (run && tid < 64)
+            {
+                const int p1 = dev::circle_at_fast(pu, 0, kW), p2 = dev::circle_at_fast(kW - 1 - pu, 0, kW);
+                gpp = g[0][p1][p1];
+                gqq = g[0][p2][p2];
+                gpq = g[0][min(p1, p2)][max(p1, p2)];
+                double c, s;
+                const bool rot = block_rotation<T>(static_cast<double>(gpp), static_cast<double>(gqq),
+                                                   static_cast<double>(gpq), eps, c, s);
+                my_c = static_cast<T>(c);
+                my_s = static_cast<T>(s);
+                const unsigned long long m = __ballot(rot);
+                if (tid < kPairs)
+                {
+                    cs[0][tid][0] = my_c;
+                    cs[0][tid][1] = my_s;
+                    my_rotations += rot ? 1 : 0;
+                }
+                if (tid == 0)
+                {
+                    rmask[0] = m;
+                }
+            }
+            __syncthreads();
             for (int rr = 0; run && rr < kW - 1; ++rr)
             {
                 const int b = rr & 1;
-                // phase 1: block loads, the round's rotations, the previous round's Q update
-                T g11 = 0, g12 = 0, g21 = 0, g22 = 0;
-                int o11 = 0, o12 = 0, o21 = 0, o22 = 0;
-                T *gf = &g[0][0];
+                T(*gc)[kW + 1] = g[b];
+                T *gn = &g[b ^ 1][0][0];
                 if (gthr)
                 {
-                    o11 = min(x1, z1) * (kW + 1) + max(x1, z1);
-                    o12 = min(x1, z2) * (kW + 1) + max(x1, z2);
-                    o21 = min(x2, z1) * (kW + 1) + max(x2, z1);
-                    o22 = min(x2, z2) * (kW + 1) + max(x2, z2);
-                    g11 = gf[o11];
-                    g12 = gf[o12];
-                    g21 = gf[o21];
-                    g22 = gf[o22];
-                }
-                if (tid < 64)
-                {
-                    double c, s;
-                    const bool rot = block_rotation<T>(static_cast<double>(g[p1][p1]), static_cast<double>(g[p2][p2]),
-                                                       static_cast<double>(g[min(p1, p2)][max(p1, p2)]), eps, c, s);
-                    const unsigned long long m = __ballot(rot);
-                    if (tid < kPairs)
-                    {
-                        cs[b][tid][0] = static_cast<T>(c);
-                        cs[b][tid][1] = static_cast<T>(s);
-                        my_rotations += rot ? 1 : 0;
-                    }
-                    if (tid == 0)
-                    {
-                        rmask[b] = m;
-                    }
-                }
-                if (qthr && rr > 0)
-                {
-                    q_round<T>(qlo, qhi, cs[b ^ 1], seg);
-                }
-                __syncthreads();
-                // phase 2: J_t^T G J_u in place (rows by J_t^T, x' = c x - s y on row pairs, then columns by J_u)
-                if (gthr)
-                {
+                    // J_t^T G J_u of this thread's block into the other buffer (copied when neither pair rotates)
+                    const int o11 = min(x1, z1) * (kW + 1) + max(x1, z1);
+                    const int o12 = min(x1, z2) * (kW + 1) + max(x1, z2);
+                    const int o21 = min(x2, z1) * (kW + 1) + max(x2, z1);
+                    const int o22 = min(x2, z2) * (kW + 1) + max(x2, z2);
+                    const T *gf = &gc[0][0];
+                    T n11 = gf[o11], n12 = gf[o12], n21 = gf[o21], n22 = gf[o22];
                     const unsigned long long m = rmask[b];
                     if (((m >> bt) | (m >> bu)) & 1ull)
                     {
-                        const T ct = cs[b][bt][0], st = cs[b][bt][1];
-                        const T cu = cs[b][bu][0], su = cs[b][bu][1];
-                        const T h11 = ct * g11 - st * g21, h12 = ct * g12 - st * g22;
-                        const T h21 = st * g11 + ct * g21, h22 = st * g12 + ct * g22;
-                        gf[o11] = cu * h11 - su * h12;
-                        gf[o12] = su * h11 + cu * h12;
-                        gf[o22] = su * h21 + cu * h22;
-                        if (bt != bu)
-                        {
-                            gf[o21] = cu * h21 - su * h22; // on the diagonal block entry 21 aliases entry 12
-                        }
+                        rotate_block(cs[b][bt][0], cs[b][bt][1], cs[b][bu][0], cs[b][bu][1], n11, n12, n21, n22, n11,
+                                     n12, n21, n22);
+                    }
+                    gn[o11] = n11;
+                    gn[o12] = n12;
+                    gn[o22] = n22;
+                    if (bt != bu)
+                    {
+                        gn[o21] = n21; // on the diagonal block entry 21 aliases entry 12
                     }
                     x1 = next_column(x1);
                     x2 = next_column(x2);
                     z1 = next_column(z1);
                     z2 = next_column(z2);
                 }
-                p1 = next_column(p1);
-                p2 = next_column(p2);
+                if (qthr)
+                {
+                    q_round<T>(qlo, qhi, cs[b], seg);
+                }
+                if (tid < 64 && rr + 1 < kW - 1)
+                {
+                    // New diagonal entries of this lane's pair (its diagonal block; exact copy when not rotating).
+                    T d11, d12, d21, d22;
+                    rotate_block(my_c, my_s, my_c, my_s, gpp, gpq, gpq, gqq, d11, d12, d21, d22);
+                    // The pair of round rr+1: the columns now at positions pu and 63 - pu came from positions
+                    // pa and pb, i.e. from pairs ta (first column when pa < 32) and tb of round rr (ta != tb).
+                    const int pa = previous_position(pu), pb = previous_position(kW - 1 - pu);
+                    const int ta = pa < kPairs ? pa : kW - 1 - pa, tb = pb < kPairs ? pb : kW - 1 - pb;
+                    const bool fa = pa < kPairs, fb = pb < kPairs;
+                    const T a1 = __shfl(d11, ta, 64), a2 = __shfl(d22, ta, 64);
+                    const T b1 = __shfl(d11, tb, 64), b2 = __shfl(d22, tb, 64);
+                    const int t1 = min(ta, tb), t2 = max(ta, tb);
+                    const T c1 = __shfl(my_c, t1, 64), s1 = __shfl(my_s, t1, 64);
+                    const T c2 = __shfl(my_c, t2, 64), s2 = __shfl(my_s, t2, 64);
+                    // Their new off-diagonal entry: block (t1, t2) of round rr, as the G waves compute it.
+                    const int r1 = dev::circle_at_fast(t1, rr, kW), r2 = dev::circle_at_fast(kW - 1 - t1, rr, kW);
+                    const int k1 = dev::circle_at_fast(t2, rr, kW), k2 = dev::circle_at_fast(kW - 1 - t2, rr, kW);
+                    T f11, f12, f21, f22;
+                    rotate_block(c1, s1, c2, s2, gc[min(r1, k1)][max(r1, k1)], gc[min(r1, k2)][max(r1, k2)],
+                                 gc[min(r2, k1)][max(r2, k1)], gc[min(r2, k2)][max(r2, k2)], f11, f12, f21, f22);
+                    const bool row_first = (ta == t1) ? fa : fb; // role of the column from pair t1
+                    const bool col_first = (ta == t1) ? fb : fa; // role of the column from pair t2
+                    gpp = fa ? a1 : a2;
+                    gqq = fb ? b1 : b2;
+                    gpq = row_first ? (col_first ? f11 : f12) : (col_first ? f21 : f22);
+                    double c, s;
+                    const bool rot = block_rotation<T>(static_cast<double>(gpp), static_cast<double>(gqq),
+                                                       static_cast<double>(gpq), eps, c, s);
+                    my_c = static_cast<T>(c);
+                    my_s = static_cast<T>(s);
+                    const unsigned long long m = __ballot(rot);
+                    if (tid < kPairs)
+                    {
+                        cs[b ^ 1][tid][0] = my_c;
+                        cs[b ^ 1][tid][1] = my_s;
+                        my_rotations += rot ? 1 : 0;
+                    }
+                    if (tid == 0)
+                    {
+                        rmask[b ^ 1] = m;
+                    }
+                }
                 __syncthreads();
-            }
-            if (run && qthr)
-            {
-                q_round<T>(qlo, qhi, cs[(kW - 2) & 1], seg); // the last round's update
             }
             if (tid < kPairs && my_rotations > 0)
             {
@@ -437,7 +514,8 @@ namespace jsvd
             // norm ordering of the next sweep
             if (norm2 != nullptr && tid < kW)
             {
-                norm2[pair_column(perm, I, J, tid)] = static_cast<double>(g[tid][tid]);
+                // the last inner round writes buffer (kW - 1) & 1
+                norm2[pair_column(perm, I, J, tid)] = static_cast<double>(g[run ? ((kW - 1) & 1) : 0][tid][tid]);
             }
             if (tid == 0)
             {
@@ -458,17 +536,48 @@ namespace jsvd
             }
         }
 
-        // Y = X Q for one row slab of [X_I X_J]; grid.y spans the slabs of A followed by those of V.
+        // Row-slab height of the apply kernel: 128 rows in FP32, 64 in FP64 (LDS for two workgroups per CU).
+        template <typename T>
+        constexpr int apply_rows() { return sizeof(T) == 4 ? 128 : 64; }
+
+        template <typename T>
+        struct Vec16; // 16-byte vector of T
+        typedef float f32x4v __attribute__((ext_vector_type(4)));
+        typedef double f64x2v __attribute__((ext_vector_type(2)));
+        template <>
+        struct Vec16<float>
+        {
+            using type = f32x4v;
+        };
+        template <>
+        struct Vec16<double>
+        {
+            using type = f64x2v;
+        };
+
+        // Y = X Q for the row slabs of [X_I X_J] (the slabs of A, then those of V, apply_rows<T>() rows each).
+        // Workgroup y of a pair takes slabs y, y + gridDim.y, ...: Q is read once per workgroup (FP32: kept in
+        // registers as MFMA B operands; FP64: in LDS), the next slab's global loads are in flight while the
+        // current slab's MFMAs run, and the product is stored straight from the accumulators.
         template <typename T>
         __global__ __launch_bounds__(256) void block_apply_kernel(T *__restrict__ a, int64_t lda, int slabs_a,
-                                                                  T *__restrict__ v, int64_t ldv, int nb, int Nb,
-                                                                  int round, const T *__restrict__ q_in,
+                                                                  T *__restrict__ v, int64_t ldv, int slabs_v,
+                                                                  int nb, int Nb, int round,
+                                                                  const T *__restrict__ q_in,
                                                                   const int *__restrict__ apply_flag,
                                                                   const int *__restrict__ perm,
                                                                   const int *__restrict__ skip_if_zero)
         {
-            __shared__ T xs[kW][kH + 1];
-            __shared__ T qs[kW][kW + 1];
+            constexpr int H = apply_rows<T>();
+            constexpr int kVec = 16 / static_cast<int>(sizeof(T)); // elements per 16-byte vector
+            constexpr int kLd = H + kVec;                           // LDS row stride: 16-byte aligned rows
+            constexpr int kColVecs = H / kVec;                      // vectors per column segment
+            constexpr int kPer = kW * kColVecs / 256;               // vectors per thread per slab
+            constexpr int kRT = H / 64;                             // 16-row tiles per wave
+            constexpr bool kQRegs = sizeof(T) == 4;
+            using V = typename Vec16<T>::type;
+            __shared__ __align__(16) T xs[kW][kLd];
+            __shared__ T qs[kQRegs ? 1 : kW][kW + 1];
             if (skip_if_zero != nullptr && *skip_if_zero == 0)
             {
                 return;
@@ -480,63 +589,153 @@ namespace jsvd
             {
                 return;
             }
-            const int sy = blockIdx.y;
-            T *x = (sy < slabs_a) ? a : v;
-            const int64_t ld = (sy < slabs_a) ? lda : ldv;
-            const int64_t r0 = static_cast<int64_t>((sy < slabs_a) ? sy : sy - slabs_a) * kH;
-            load_slab(x, ld, perm, I, J, r0, xs);
-            const T *qg = q_in + static_cast<int64_t>(slot) * (kW * kW);
-            for (int e = threadIdx.x; e < kW * kW; e += 256)
+            const int total = slabs_a + slabs_v;
+            int s = blockIdx.y;
+            if (s >= total)
             {
-                qs[e / kW][e % kW] = qg[e];
+                return;
             }
-            __syncthreads();
-            const int lane = threadIdx.x % 64;
-            const int wave = threadIdx.x / 64;
-            // wave computes output rows [32*wave, 32*wave+32) (two 16-row tiles) x all 64 columns (four tiles)
-            typename Mfma<T>::Acc acc[2][4];
-            for (int a2 = 0; a2 < 2; ++a2)
-            {
-                for (int t = 0; t < 4; ++t)
-                {
-                    acc[a2][t] = typename Mfma<T>::Acc{0, 0, 0, 0};
-                }
-            }
-            for (int c0 = 0; c0 < kW; c0 += 4)
-            {
-                const int kc = c0 + lane / 16;
-                const T av0 = xs[kc][32 * wave + lane % 16];
-                const T av1 = xs[kc][32 * wave + 16 + lane % 16];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                {
-                    const T bv = qs[kc][16 * t + lane % 16];
-                    acc[0][t] = Mfma<T>::step(av0, bv, acc[0][t]);
-                    acc[1][t] = Mfma<T>::step(av1, bv, acc[1][t]);
-                }
-            }
-            __syncthreads(); // every wave has read xs; overwrite it with the product
-#pragma unroll
-            for (int a2 = 0; a2 < 2; ++a2)
+            const int tid = threadIdx.x;
+            const int lane = tid % 64;
+            const int wave = tid / 64;
+            const T *qg = q_in + static_cast<int64_t>(slot) * (kW * kW); // row-major Q[c][j]
+            T bq[kQRegs ? kW / 4 : 1][4];
+            if constexpr (kQRegs)
             {
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int kk = 0; kk < kW / 4; ++kk)
                 {
 #pragma unroll
-                    for (int vv = 0; vv < 4; ++vv)
+                    for (int t = 0; t < 4; ++t)
                     {
-                        const int r = 32 * wave + 16 * a2 + Mfma<T>::row(lane, vv);
-                        const int c = 16 * t + lane % 16;
-                        xs[c][r] = acc[a2][t][vv];
+                        bq[kk][t] = qg[(4 * kk + lane / 16) * kW + 16 * t + lane % 16];
                     }
                 }
             }
-            __syncthreads();
-            for (int e = threadIdx.x; e < kW * kH; e += 256)
+            else
             {
-                const int c = e / kH;
-                const int r = e % kH;
-                x[pair_column(perm, I, J, c) * ld + r0 + r] = xs[c][r];
+                for (int e = tid; e < kW * kW; e += 256)
+                {
+                    qs[e / kW][e % kW] = qg[e];
+                }
+            }
+            // This thread's vectors of a slab: column (tid + 256 k) / kColVecs, rows ((tid + 256 k) % kColVecs) * kVec
+            // .. + kVec; the row offset is the same for every k (256 is a multiple of kColVecs).
+            static_assert(256 % kColVecs == 0, "apply load layout");
+            int col_of[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                col_of[k] = static_cast<int>(pair_column(perm, I, J, (tid + 256 * k) / kColVecs));
+            }
+            const int vrow = (tid % kColVecs) * kVec;
+            V pre[kPer];
+            {
+                const bool in_a = s < slabs_a;
+                const T *x = in_a ? a : v;
+                const int64_t ld = in_a ? lda : ldv;
+                const int64_t r0 = static_cast<int64_t>(in_a ? s : s - slabs_a) * H + vrow;
+#pragma unroll
+                for (int k = 0; k < kPer; ++k)
+                {
+                    pre[k] = *reinterpret_cast<const V *>(x + static_cast<int64_t>(col_of[k]) * ld + r0);
+                }
+            }
+            bool first = true;
+            for (; s < total; s += gridDim.y)
+            {
+                if (!first)
+                {
+                    __syncthreads(); // every wave has finished reading xs
+                }
+                first = false;
+#pragma unroll
+                for (int k = 0; k < kPer; ++k)
+                {
+                    *reinterpret_cast<V *>(&xs[(tid + 256 * k) / kColVecs][vrow]) = pre[k];
+                }
+                __syncthreads();
+                const bool in_a = s < slabs_a;
+                T *x = in_a ? a : v;
+                const int64_t ld = in_a ? lda : ldv;
+                const int64_t r0 = static_cast<int64_t>(in_a ? s : s - slabs_a) * H;
+                {
+                    // next slab of this workgroup (the last iteration re-reads the current slab, unused; this keeps
+                    // the prefetch unconditional so `pre` stays in registers)
+                    const int sn = (s + static_cast<int>(gridDim.y) < total) ? s + static_cast<int>(gridDim.y) : s;
+                    const bool n_in_a = sn < slabs_a;
+                    const T *xn = n_in_a ? a : v;
+                    const int64_t ldn = n_in_a ? lda : ldv;
+                    const int64_t r0n = static_cast<int64_t>(n_in_a ? sn : sn - slabs_a) * H + vrow;
+#pragma unroll
+                    for (int k = 0; k < kPer; ++k)
+                    {
+                        pre[k] = *reinterpret_cast<const V *>(xn + static_cast<int64_t>(col_of[k]) * ldn + r0n);
+                    }
+                }
+                // wave: output rows [wave * 16 kRT, (wave + 1) * 16 kRT) x all 64 columns
+                typename Mfma<T>::Acc acc[kRT][4];
+#pragma unroll
+                for (int a2 = 0; a2 < kRT; ++a2)
+                {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                    {
+                        acc[a2][t] = typename Mfma<T>::Acc{0, 0, 0, 0};
+                    }
+                }
+                const int rb = wave * 16 * kRT;
+#pragma unroll
+                for (int kk = 0; kk < kW / 4; ++kk)
+                {
+                    const int kc = 4 * kk + lane / 16;
+                    T av[kRT];
+#pragma unroll
+                    for (int a2 = 0; a2 < kRT; ++a2)
+                    {
+                        av[a2] = xs[kc][rb + 16 * a2 + lane % 16];
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                    {
+                        T bv;
+                        if constexpr (kQRegs)
+                        {
+                            bv = bq[kk][t];
+                        }
+                        else
+                        {
+                            bv = qs[kc][16 * t + lane % 16];
+                        }
+#pragma unroll
+                        for (int a2 = 0; a2 < kRT; ++a2)
+                        {
+                            acc[a2][t] = Mfma<T>::step(av[a2], bv, acc[a2][t]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                {
+                    T *col = x + pair_column(perm, I, J, 16 * t + lane % 16) * ld + r0 + rb;
+#pragma unroll
+                    for (int a2 = 0; a2 < kRT; ++a2)
+                    {
+                        if constexpr (sizeof(T) == 4)
+                        {
+                            // rows 4 (lane / 16) .. + 3 of the tile: one 16-byte store
+                            *reinterpret_cast<f32x4v *>(col + 16 * a2 + 4 * (lane / 16)) = acc[a2][t];
+                        }
+                        else
+                        {
+#pragma unroll
+                            for (int vv = 0; vv < 4; ++vv)
+                            {
+                                col[16 * a2 + Mfma<T>::row(lane, vv)] = acc[a2][t][vv];
+                            }
+                        }
+                    }
+                }
             }
         }
 
@@ -671,9 +870,13 @@ namespace jsvd
                            static_cast<const T *>(partial), parts, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
                            offmax, perm, norm2, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, g.slabs_a + g.slabs_v), dim3(256), 0, stream, a, g.lda,
-                           g.slabs_a, v, g.ldv, g.nb, g.Nb, round, static_cast<const T *>(q),
-                           static_cast<const int *>(apply_flag), perm, skip_if_zero);
+        // lda and ldv are multiples of kH, hence of the apply slab height; about two workgroups per CU in total
+        const int h = apply_rows<T>();
+        const int sa = static_cast<int>(g.lda / h), sv = static_cast<int>(g.ldv / h);
+        const int per_pair = std::max(1, std::min(sa + sv, (512 + pairs - 1) / pairs));
+        hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, per_pair), dim3(256), 0, stream, a, g.lda, sa, v, g.ldv,
+                           sv, g.nb, g.Nb, round, static_cast<const T *>(q), static_cast<const int *>(apply_flag),
+                           perm, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
     }
 
