@@ -39,10 +39,15 @@ namespace jsvd
         }
         [[nodiscard]] int rounds() const { return Nb - 1; }
         [[nodiscard]] int pairs() const { return Nb / 2; }
+        // Elements of the partial-Gram buffer: at most one partial per kH-row slab.
+        [[nodiscard]] size_t partial_elems() const
+        {
+            return static_cast<size_t>(pairs()) * static_cast<size_t>(slabs_a) * kW * kW;
+        }
     };
 
     // One block round (Gram partials, reduction + inner sweep, apply to A and V).  partial holds
-    // pairs * slabs_a * kW * kW elements, q pairs * kW * kW, apply_flag pairs ints.
+    // g.partial_elems() elements, q pairs * kW * kW, apply_flag pairs ints.
     // perm maps the sweep's logical column order to physical columns (A and V columns are addressed
     // through it); norm2 receives the squared norm of every column after its last rotation of the sweep.
     template <typename T>

@@ -53,7 +53,7 @@ namespace jsvd
         cap_ = std::max(cfg_.max_sweeps, mixed_.fp32_max_sweeps);
         const size_t a_elems = static_cast<size_t>(g_.lda) * g_.n_pad;
         const size_t v_elems = static_cast<size_t>(g_.ldv) * g_.n_pad;
-        const size_t part_elems = static_cast<size_t>(g_.pairs()) * g_.slabs_a * kW * kW;
+        const size_t part_elems = g_.partial_elems();
         cap_a_elems_ = a_elems;
         cap_v_elems_ = v_elems;
         cap_part_elems_ = part_elems;
@@ -204,7 +204,7 @@ namespace jsvd
         const BlockGeometry g = BlockGeometry::make(m, n);
         const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
         const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
-        const size_t part_elems = static_cast<size_t>(g.pairs()) * g.slabs_a * kW * kW;
+        const size_t part_elems = g.partial_elems();
         if (g.nb < 2 || a_elems > cap_a_elems_ || v_elems > cap_v_elems_ || part_elems > cap_part_elems_ ||
             g.pairs() > cap_pairs_ || g.n_pad > cap_npad_)
         {
