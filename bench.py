@@ -106,7 +106,8 @@ def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
     acol[:, :m] = torch.from_numpy(a_host.T.copy()).to("cuda")
     vcol = torch.zeros((n, ldv), dtype=torch.float64, device="cuda")
     vcol[:, :n] = torch.eye(n, dtype=torch.float64, device="cuda")
-    rot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    slots = 64  # the solver's counter spreading (one slot per 8 of the 512 pairs)
+    rot = torch.zeros(slots, dtype=torch.int32, device="cuda")
     N = n + (n % 2)
     rounds = N - 1
     launches = min(launches, rounds)
@@ -114,14 +115,13 @@ def round_kernel_roofline(torch, jsvd, a_host, eps, launches=255):
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     jsvd.round_(acol, lda, m, vcol, ldv, n, 0, eps, stream=stream)   # warm the code path (rotates round 0)
-    rot.zero_()
     torch.cuda.synchronize()
     t0.record(stream)
-    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, eps, rot_count=rot, stream=stream)
+    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, eps, rot_count=rot, stream=stream, count_slots=slots)
     t1.record(stream)
     torch.cuda.synchronize()
     total_ms = t0.elapsed_time(t1)
-    rotated = int(rot.item())
+    rotated = int(rot.sum().item())
     pairs = launches * (n // 2)
     # fused kernel: every pair reads its two A columns (Gram); a rotated pair also writes them and
     # reads + writes its two V columns.
@@ -138,18 +138,18 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
     acol = torch.randn((n, lda), dtype=torch.float64, device="cuda", generator=gen)
     acol *= torch.pow(10.0, -12.0 * torch.arange(n, device="cuda", dtype=torch.float64) / (n - 1)).unsqueeze(1)
     vcol = torch.eye(n, dtype=torch.float64, device="cuda")
-    rot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    rot = torch.zeros(64, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     starts = [torch.cuda.Event(enable_timing=True)]
     ends = [torch.cuda.Event(enable_timing=True)]
     jsvd.round_(acol, lda, m, vcol, ldv, n, 0, 1e-15, stream=stream)   # warm the code path
     torch.cuda.synchronize()
     starts[0].record(stream)
-    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, 1e-15, rot_count=rot, stream=stream)
+    jsvd.rounds(acol, lda, m, vcol, ldv, n, 1, launches, 1e-15, rot_count=rot, stream=stream, count_slots=64)
     ends[0].record(stream)
     torch.cuda.synchronize()
     total_ms = starts[0].elapsed_time(ends[0])
-    rotated = int(rot.item())
+    rotated = int(rot.sum().item())
     pairs = launches * (n // 2)
     algo_bytes = 8 * (pairs * 2 * m + rotated * (2 * m + 4 * n)) / launches
     avg_us = 1000.0 * total_ms / launches

@@ -137,7 +137,7 @@ void jsvd_mixed_config_default(jsvd_mixed_config *mixed);
 
 /*
  * Plan for the path selected by mixed->path (NULL: the plain FP64 path, identical to jsvd_plan_create).
- * The block paths need n >= 64 (two 32-column blocks); smaller n is rejected with
+ * The block paths need at least two 32-column blocks (n >= 33); smaller n is rejected with
  * JSVD_ERR_INVALID_ARGUMENT.
  */
 int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed, void *stream,
@@ -368,10 +368,13 @@ int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, in
 /*
  * `rounds` consecutive fused rounds (rounds first_round, first_round + 1, ... modulo the n - 1 (even n) or n
  * (odd n) rounds of a sweep), launched back to back on `stream` with the arguments of jsvd_round_f64; rounds =
- * n - 1 from round 0 is one sweep of the round loop of src/kernels.cu:893-966.
+ * n - 1 from round 0 is one sweep of the round loop of src/kernels.cu:893-966.  rot_count and offmax (each may
+ * be NULL) hold count_slots entries (a power of two, 1..4096): the pair of slot k of a round counts into entry
+ * k % count_slots, as the solver does to keep same-address atomics apart; the caller adds the entries (takes
+ * their maximum for offmax).
  */
 int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t first_round,
-                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax,
+                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax, int32_t count_slots,
                     const int32_t *skip_if_zero, void *stream);
 
 /*

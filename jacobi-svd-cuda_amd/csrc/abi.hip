@@ -120,6 +120,10 @@ struct jsvd_plan
     double *v_col = nullptr;            // ldv x n column-major V
     int *rot_count = nullptr;           // [max_sweeps] rotated pairs per sweep
     unsigned long long *offmax = nullptr; // [max_sweeps] max pair cosine per sweep (double bits)
+    // per-sweep counter slots the round kernels update (folded into rot_count / offmax after each sweep)
+    static constexpr int kCountSlots = 64;
+    int *rot_slots = nullptr;              // [max_sweeps][kCountSlots]
+    unsigned long long *off_slots = nullptr; // [max_sweeps][kCountSlots]
     int *h_rot = nullptr;               // pinned mirror of rot_count
     hipEvent_t sweep_done[2] = {nullptr, nullptr};
     // Host-API staging (allocated on first host solve).
@@ -143,6 +147,8 @@ struct jsvd_plan
         (void)hipFree(v_col);
         (void)hipFree(rot_count);
         (void)hipFree(offmax);
+        (void)hipFree(rot_slots);
+        (void)hipFree(off_slots);
         (void)hipHostFree(h_rot);
         (void)hipFree(d_a);
         (void)hipFree(d_u);
@@ -205,6 +211,8 @@ struct jsvd_plan
         JSVD_HIP_CHECK(hipMalloc(&v_col, sizeof(double) * ldv * n));
         JSVD_HIP_CHECK(hipMalloc(&rot_count, sizeof(int) * cfg.max_sweeps));
         JSVD_HIP_CHECK(hipMalloc(&offmax, sizeof(unsigned long long) * cfg.max_sweeps));
+        JSVD_HIP_CHECK(hipMalloc(&rot_slots, sizeof(int) * kCountSlots * cfg.max_sweeps));
+        JSVD_HIP_CHECK(hipMalloc(&off_slots, sizeof(unsigned long long) * kCountSlots * cfg.max_sweeps));
         JSVD_HIP_CHECK(hipHostMalloc(&h_rot, sizeof(int) * cfg.max_sweeps, hipHostMallocDefault));
         for (hipEvent_t &e : sweep_done)
         {
@@ -277,6 +285,8 @@ struct jsvd_plan
         jsvd::launch_identity(v_col, ldv, n, stream);
         JSVD_HIP_CHECK(hipMemsetAsync(rot_count, 0, sizeof(int) * max_sweeps, stream));
         JSVD_HIP_CHECK(hipMemsetAsync(offmax, 0, sizeof(unsigned long long) * max_sweeps, stream));
+        JSVD_HIP_CHECK(hipMemsetAsync(rot_slots, 0, sizeof(int) * kCountSlots * max_sweeps, stream));
+        JSVD_HIP_CHECK(hipMemsetAsync(off_slots, 0, sizeof(unsigned long long) * kCountSlots * max_sweeps, stream));
         std::fill(h_rot, h_rot + max_sweeps, -1);
 
         const int N = static_cast<int>(n + (n & 1));
@@ -290,9 +300,11 @@ struct jsvd_plan
             const int *skip = (s > 0) ? rot_count + (s - 1) : nullptr;
             for (int r = 0; r < rounds; ++r)
             {
-                jsvd::launch_round(shape, a_col, lda, m, v_col, ldv, n, r, eps, rot_count + s, offmax + s, skip,
-                                   stream);
+                jsvd::launch_round(shape, a_col, lda, m, v_col, ldv, n, r, eps, rot_slots + s * kCountSlots,
+                                   off_slots + s * kCountSlots, skip, stream, kCountSlots);
             }
+            jsvd::launch_sweep_totals(rot_slots + s * kCountSlots, off_slots + s * kCountSlots, kCountSlots,
+                                      rot_count + s, offmax + s, stream);
             JSVD_HIP_CHECK(hipMemcpyAsync(h_rot + s, rot_count + s, sizeof(int), hipMemcpyDeviceToHost, stream));
             JSVD_HIP_CHECK(hipEventRecord(sweep_done[s & 1], stream));
             enqueued = s + 1;
@@ -718,10 +730,12 @@ int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, in
 }
 
 int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t first_round,
-                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax,
+                    int32_t rounds, double epsilon, int32_t *rot_count, uint64_t *offmax, int32_t count_slots,
                     const int32_t *skip_if_zero, void *stream)
 {
     return jsvd::guarded([&] {
+        require(count_slots >= 1 && count_slots <= 4096 && (count_slots & (count_slots - 1)) == 0,
+                "count_slots must be a power of two in [1, 4096].");
         require(m > 0 && n > 0 && m >= n, "invalid shape (need m >= n > 0).");
         require(lda >= m && lda % 2 == 0, "lda must be even and >= m.");
         require(ldv >= n && ldv % 2 == 0, "ldv must be even and >= n.");
@@ -740,7 +754,7 @@ int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, i
         {
             const int round = static_cast<int>((first_round + k) % (N - 1));
             jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
-                               reinterpret_cast<unsigned long long *>(offmax), skip_if_zero, s);
+                               reinterpret_cast<unsigned long long *>(offmax), skip_if_zero, s, count_slots);
         }
     });
 }

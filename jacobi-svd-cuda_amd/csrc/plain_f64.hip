@@ -180,9 +180,12 @@ namespace jsvd
                                                           double *__restrict__ v, int64_t ldv, int64_t n, int N,
                                                           int round, double eps, int *__restrict__ rot_count,
                                                           unsigned long long *__restrict__ offmax,
-                                                          const int *__restrict__ skip_if_zero)
+                                                          const int *__restrict__ skip_if_zero, int count_slots)
         {
             __shared__ double scratch[T / 64 > 0 ? T / 64 : 1][3];
+            // the pair's slot of the rotation counter and the cosine maximum (count_slots is a power of two):
+            // spreading the one atomic per workgroup over slots keeps same-address atomics from serialising
+            const int cslot = static_cast<int>(blockIdx.x) & (count_slots - 1);
             if (skip_if_zero != nullptr && *skip_if_zero == 0)
             {
                 return;
@@ -264,11 +267,11 @@ namespace jsvd
                 {
                     if (offmax)
                     {
-                        dev::atomic_max_nonneg(offmax, dev::pair_cosine(spp, sqq, spq));
+                        dev::atomic_max_nonneg(offmax + cslot, dev::pair_cosine(spp, sqq, spq));
                     }
                     if (rotate && rot_count)
                     {
-                        atomicAdd(rot_count, 1);
+                        atomicAdd(rot_count + cslot, 1);
                     }
                 }
                 if (!rotate)
@@ -327,11 +330,11 @@ namespace jsvd
                 {
                     if (offmax)
                     {
-                        dev::atomic_max_nonneg(offmax, dev::pair_cosine(spp, sqq, spq));
+                        dev::atomic_max_nonneg(offmax + cslot, dev::pair_cosine(spp, sqq, spq));
                     }
                     if (rotate && rot_count)
                     {
-                        atomicAdd(rot_count, 1);
+                        atomicAdd(rot_count + cslot, 1);
                     }
                 }
                 if (!rotate)
@@ -340,6 +343,32 @@ namespace jsvd
                 }
                 rotate_stream(colp, colq, m2, odd_tail, c, s, tid, T);
                 rotate_stream(vp, vq, n2, odd_tail_v, c, s, tid, T);
+            }
+        }
+
+        // Folds the counter slots of one sweep: one 64-lane wave, slots <= 64 x 64.
+        __global__ __launch_bounds__(64) void sweep_totals_kernel(const int *__restrict__ rot_slots,
+                                                                  const unsigned long long *__restrict__ off_slots,
+                                                                  int slots, int *__restrict__ rot_total,
+                                                                  unsigned long long *__restrict__ off_total)
+        {
+            int count = 0;
+            unsigned long long best = 0; // non-negative doubles order like their bit patterns
+            for (int k = threadIdx.x; k < slots; k += 64)
+            {
+                count += rot_slots[k];
+                best = max(best, off_slots[k]);
+            }
+#pragma unroll
+            for (int offset = 32; offset > 0; offset >>= 1)
+            {
+                count += __shfl_xor(count, offset, 64);
+                best = max(best, static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(best), offset, 64)));
+            }
+            if (threadIdx.x == 0)
+            {
+                *rot_total = count;
+                *off_total = best;
             }
         }
 
@@ -496,7 +525,7 @@ namespace jsvd
         }
 
         using RoundFn = void (*)(double *, int64_t, int64_t, double *, int64_t, int64_t, int, int, double, int *,
-                                 unsigned long long *, const int *);
+                                 unsigned long long *, const int *, int);
 
         // Instantiated (EA, EV) pairs; EV <= EA because n <= m.  EV == 0 streams V.
         template <int T>
@@ -563,11 +592,15 @@ namespace jsvd
 
     void launch_round(const RoundShape &shape, double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n,
                       int round, double eps, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
-                      hipStream_t stream)
+                      hipStream_t stream, int count_slots)
     {
         if (n < 2)
         {
             return;
+        }
+        if (count_slots < 1 || (count_slots & (count_slots - 1)) != 0)
+        {
+            throw_invalid("count_slots must be a power of two.");
         }
         const int N = static_cast<int>(n + (n & 1));
         RoundFn fn = nullptr;
@@ -578,7 +611,15 @@ namespace jsvd
         default: fn = pick_round<1024>(shape.ea, shape.ev); break;
         }
         hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(N / 2)), dim3(static_cast<unsigned>(shape.threads)), 0,
-                           stream, a, lda, m, v, ldv, n, N, round, eps, rot_count, offmax, skip_if_zero);
+                           stream, a, lda, m, v, ldv, n, N, round, eps, rot_count, offmax, skip_if_zero, count_slots);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    void launch_sweep_totals(const int *rot_slots, const unsigned long long *off_slots, int slots, int *rot_total,
+                             unsigned long long *off_total, hipStream_t stream)
+    {
+        hipLaunchKernelGGL(sweep_totals_kernel, dim3(1), dim3(64), 0, stream, rot_slots, off_slots, slots, rot_total,
+                           off_total);
         JSVD_HIP_CHECK(hipGetLastError());
     }
 

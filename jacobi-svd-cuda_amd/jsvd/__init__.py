@@ -185,7 +185,7 @@ def lib() -> ctypes.CDLL:
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_block_eigen.argtypes = [ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
-                                      ctypes.c_double, _vp, _vp, _vp, _vp]
+                                      ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp, _vp]
         for name in declared_symbols():
             if name not in ("jsvd_config_default", "jsvd_last_error", "jsvd_version", "jsvd_mixed_config_default"):
                 getattr(L, name).restype = ctypes.c_int
@@ -370,11 +370,15 @@ def round_(a_col, lda: int, m: int, v_col, ldv: int, n: int, rnd: int, eps: floa
 
 
 def rounds(a_col, lda: int, m: int, v_col, ldv: int, n: int, first_round: int, count: int, eps: float,
-           rot_count=None, offmax=None, skip_if_zero=None, stream=None) -> None:
-    """`count` consecutive fused rounds from `first_round`, launched back to back (jsvd_rounds_f64)."""
+           rot_count=None, offmax=None, skip_if_zero=None, stream=None, count_slots: int = 1) -> None:
+    """`count` consecutive fused rounds from `first_round`, launched back to back (jsvd_rounds_f64); rot_count
+    and offmax hold count_slots entries each (summed / maximised by the caller)."""
+    for t in (rot_count, offmax):
+        if t is not None and t.numel() < count_slots:
+            raise ValueError("counter tensors need count_slots entries")
     check(lib().jsvd_rounds_f64(_ptr(a_col), lda, m, _ptr(v_col), ldv, n, first_round, count, eps,
                                 _ptr(rot_count) if rot_count is not None else None,
-                                _ptr(offmax) if offmax is not None else None,
+                                _ptr(offmax) if offmax is not None else None, int(count_slots),
                                 _ptr(skip_if_zero) if skip_if_zero is not None else None,
                                 _stream_handle(stream)))
 

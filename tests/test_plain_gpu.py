@@ -151,20 +151,25 @@ def test_rounds_equal_single_rounds(torch_cuda, m, n, first, count):
     lda, ldv = m + (m % 2), n + (n % 2)
     per_round = n - 1 + (n % 2)
     results = []
-    for batched in (False, True):
+    for mode in ("single", "batched", "slots"):
         d_a = _dev(torch, _col_padded(a, lda))
         d_v = _dev(torch, _col_padded(np.eye(n), ldv))
-        rot = torch.zeros(1, dtype=torch.int32, device="cuda")
-        if batched:
-            jsvd.rounds(d_a, lda, m, d_v, ldv, n, first, count, 1e-15, rot_count=rot)
+        rot = torch.zeros(8, dtype=torch.int32, device="cuda")
+        off = torch.zeros(8, dtype=torch.int64, device="cuda")
+        if mode == "batched":
+            jsvd.rounds(d_a, lda, m, d_v, ldv, n, first, count, 1e-15, rot_count=rot, offmax=off)
+        elif mode == "slots":  # counters spread over 8 slots: same totals
+            jsvd.rounds(d_a, lda, m, d_v, ldv, n, first, count, 1e-15, rot_count=rot, offmax=off, count_slots=8)
         else:
             for k in range(count):
-                jsvd.round_(d_a, lda, m, d_v, ldv, n, (first + k) % per_round, 1e-15, rot_count=rot)
+                jsvd.round_(d_a, lda, m, d_v, ldv, n, (first + k) % per_round, 1e-15, rot_count=rot, offmax=off)
         torch.cuda.synchronize()
-        results.append((d_a.cpu().numpy(), d_v.cpu().numpy(), int(rot.item())))
-    assert np.array_equal(results[0][0], results[1][0])
-    assert np.array_equal(results[0][1], results[1][1])
-    assert results[0][2] == results[1][2] > 0
+        results.append((d_a.cpu().numpy(), d_v.cpu().numpy(), int(rot.sum().item()), int(off.max().item())))
+    for r in results[1:]:
+        assert np.array_equal(results[0][0], r[0])
+        assert np.array_equal(results[0][1], r[1])
+        assert results[0][2] == r[2] > 0
+        assert results[0][3] == r[3] > 0
     with pytest.raises(ValueError):
         jsvd.rounds(d_a, lda, m, d_v, ldv, n, per_round, 1, 1e-15)
 
