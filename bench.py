@@ -283,6 +283,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
     ap.add_argument("--no-file", action="store_true", help="skip the streamed *.mat file leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch parameters (no jsvd.autotune before the timed legs)")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving leg")
     ap.add_argument("--serving-rate", type=float, default=1000.0)
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -304,6 +306,12 @@ def main() -> None:
     cfg = jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=max_sweeps)
     torch.cuda.init()
 
+    tune = None
+    if not args.no_autotune:
+        log(f"[bench] autotune {m}x{n}")
+        tune = jsvd.autotune(m, n, repetitions=2).as_dict()
+        log(f"[bench] autotune: {tune}")
+
     if args.roofline_only:
         eig = eigen_kernel_roofline(torch, jsvd)
         avg_us, bytes_per_launch, _, _ = round_kernel_roofline(torch, jsvd, a_host, eps)
@@ -312,7 +320,7 @@ def main() -> None:
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"eigen_avg_us": eig["avg_launch_us"], "round_avg_us": avg_us,
+        print(json.dumps({"eigen_avg_us": eig["avg_launch_us"], "round_avg_us": avg_us, "autotune": tune,
                           "round_algorithmic_bytes": bytes_per_launch, "copy_bytes": 2 * src.numel() * 8}),
               flush=True)
         return
@@ -358,6 +366,7 @@ def main() -> None:
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     roofline_plain = {
         "kernel": "round_kernel (plain FP64 fused Gram + rotation + Givens apply, one launch per round)",
+        "launch_shape": ({k: tune[k] for k in ("round_threads", "round_ea", "round_ev")} if tune else "built-in"),
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
@@ -404,6 +413,7 @@ def main() -> None:
         "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
         "file_stream_mixed": file_stream,
         "serving": serving,
+        "autotune": tune,
         "roofline": roofline,
         "roofline_plain": roofline_plain,
         "roofline_plain_hbm_resident": roofline_hbm,

@@ -155,6 +155,15 @@ int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info);
  */
 int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon);
 
+/*
+ * The launch parameters a plan currently uses: for the plain path the fused round kernel's threads per column
+ * pair and register chunk counts (0 = streamed) with apply_workgroups 0; for the block paths the apply
+ * kernel's total workgroups with the round fields 0.  They come from jsvd_autotune's cache when it measured
+ * the plan's shape, else from the built-in rules.  No reference counterpart.
+ */
+int jsvd_plan_launch_params(const jsvd_plan *plan, int32_t *round_threads, int32_t *round_ea, int32_t *round_ev,
+                            int32_t *apply_workgroups);
+
 /* One-shot host-pointer solve through the path selected by `mixed` (NULL: plain FP64). */
 int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
                    double *u, double *sigma, double *v, jsvd_solve_info *info);
@@ -310,6 +319,36 @@ typedef struct jsvd_mat_stream_report
 int jsvd_mat_stream_solve(const char *input_path, const char *output_path, const jsvd_config *config,
                           const jsvd_mixed_config *mixed, jsvd_mat_stream_report *report);
 
+/* Result of jsvd_autotune: the launch parameters chosen for one shape and the timings behind them. */
+typedef struct jsvd_tune_report
+{
+    int32_t candidates_tried;      /* timed candidates (round-kernel shapes + apply grid sizes) */
+    int32_t round_threads;         /* plain path: threads per column pair of the fused round kernel */
+    int32_t round_ea;              /* plain path: register chunks per thread for an A column (0 = streamed) */
+    int32_t round_ev;              /* plain path: register chunks per thread for a V column (0 = streamed) */
+    int32_t apply_workgroups;      /* block paths: total workgroups of one apply launch */
+    int32_t reserved;
+    double round_us_default;       /* us per round of the shape chosen by the built-in rule */
+    double round_us_best;          /* us per round of the chosen shape */
+    double block_round_us_default; /* us per FP32 block round with 512 apply workgroups (0 if n <= 32) */
+    double block_round_us_best;    /* us per FP32 block round with the chosen count (0 if n <= 32) */
+} jsvd_tune_report;
+
+/*
+ * Autotuner of the launch parameters for m x n (m >= n >= 1): times on the device, on seeded data of that
+ * shape, every round-kernel launch shape that suits it (16 rounds of a first sweep each) and the block apply
+ * kernel's grid size (256 / 512 / 1024 workgroups, 8 FP32 block rounds each), keeping the best of
+ * `repetitions` timings per candidate.  The winners are cached for the process: plans created or reshaped
+ * to m x n afterwards use them, except that layout_transpose_mode FORCE_ENABLE / FORCE_DISABLE still fix the
+ * round kernel's shape.  stream may be NULL (a private stream).  No counterpart in the reference, whose only
+ * tuning is the layout-transpose threshold search (src/kernels.cu:1078-1127; auto_tune_layout_transpose_threshold
+ * of include/kernels.hpp here).
+ */
+int jsvd_autotune(int64_t m, int64_t n, int32_t repetitions, void *stream, jsvd_tune_report *report);
+
+/* Forgets every cached jsvd_autotune result (plans created afterwards use the built-in rules). */
+int jsvd_autotune_clear(void);
+
 /*
  * Host evaluation of the closed-form round-robin schedule the kernels use: writes the (p, q) pairs of
  * round `round` of n columns into pairs_out (2 * (n+1)/2 int32 capacity), in slot order, dropping the
@@ -360,7 +399,8 @@ int jsvd_pair_gram_f64(const double *a, int64_t lda, int64_t m, const int32_t *p
  * (src/kernels.cu:908-965).  rot_count (device int32, may be NULL) is incremented once per rotated
  * pair; offmax (device uint64 holding a non-negative double's bits, may be NULL) receives the max of
  * |gamma|/sqrt(alpha*beta).  If skip_if_zero is non-NULL and *skip_if_zero == 0 on entry the kernel
- * does nothing (device-side early exit used after convergence).
+ * does nothing (device-side early exit used after convergence).  The launch shape is jsvd_autotune's choice
+ * for m x n when it measured one, else the built-in rule.
  */
 int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n, int32_t round,
                    double epsilon, int32_t *rot_count, uint64_t *offmax, const int32_t *skip_if_zero, void *stream);

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "autotune.hpp"
 #include "common.hpp"
 #include "mixed.hpp"
 #include "plain_f64.hpp"
@@ -93,6 +94,26 @@ namespace jsvd
         }
         return n >= cfg.layout_transpose_min_columns &&
                static_cast<uint64_t>(m) * static_cast<uint64_t>(n) >= cfg.layout_transpose_min_elements;
+    }
+
+    // Round-kernel launch shape for a plan: the autotuner's measured choice when it has one for this shape and
+    // the layout mode is AUTO, else the built-in rule.
+    RoundShape plan_round_shape(int64_t m, int64_t n, const jsvd_config &cfg)
+    {
+        Tuning t;
+        if (cfg.layout_transpose_mode == JSVD_LAYOUT_AUTO && lookup_tuning(m, n, t))
+        {
+            return t.round;
+        }
+        return choose_round_shape(m, n, use_wide_shape(m, n, cfg));
+    }
+
+    // Launch shape of the stand-alone round entry points: the autotuner's choice for m x n, else the built-in
+    // rule for one wavefront per pair where it fits.
+    RoundShape kernel_round_shape(int64_t m, int64_t n)
+    {
+        Tuning t;
+        return lookup_tuning(m, n, t) ? t.round : choose_round_shape(m, n, false);
     }
 
     int normalize_threads_per_block(int raw)
@@ -197,7 +218,7 @@ struct jsvd_plan
         cfg = config;
         lda = jsvd::round_up(m, 2);
         ldv = jsvd::round_up(n, 2);
-        shape = jsvd::choose_round_shape(m, n, jsvd::use_wide_shape(m, n, cfg));
+        shape = jsvd::plan_round_shape(m, n, cfg);
         if (user_stream)
         {
             stream = user_stream;
@@ -246,7 +267,7 @@ struct jsvd_plan
             JSVD_HIP_CHECK(hipMemsetAsync(a_col, 0, sizeof(double) * new_lda * cols, stream));
             lda = new_lda;
             ldv = jsvd::round_up(cols, 2);
-            shape = jsvd::choose_round_shape(rows, cols, jsvd::use_wide_shape(rows, cols, cfg));
+            shape = jsvd::plan_round_shape(rows, cols, cfg);
         }
         m = rows;
         n = cols;
@@ -614,6 +635,20 @@ int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon)
     });
 }
 
+int jsvd_plan_launch_params(const jsvd_plan *plan, int32_t *round_threads, int32_t *round_ea, int32_t *round_ev,
+                            int32_t *apply_workgroups)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr && round_threads && round_ea && round_ev && apply_workgroups,
+                "plan and outputs must not be null.");
+        const bool blk = plan->block != nullptr;
+        *round_threads = blk ? 0 : plan->shape.threads;
+        *round_ea = blk ? 0 : plan->shape.ea;
+        *round_ev = blk ? 0 : plan->shape.ev;
+        *apply_workgroups = blk ? plan->block->geometry().apply_wgs : 0;
+    });
+}
+
 int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info)
 {
     return jsvd::guarded([&] {
@@ -722,7 +757,7 @@ int jsvd_round_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, in
         const int64_t N = n + (n & 1);
         require(n < 2 || (round >= 0 && round < N - 1), "round out of range.");
         require(epsilon > 0.0, "epsilon must be positive.");
-        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n, false);
+        const jsvd::RoundShape shape = jsvd::kernel_round_shape(m, n);
         jsvd::launch_round(shape, a, lda, m, v, ldv, n, round, epsilon, rot_count,
                            reinterpret_cast<unsigned long long *>(offmax), skip_if_zero,
                            static_cast<hipStream_t>(stream));
@@ -748,7 +783,7 @@ int jsvd_rounds_f64(double *a, int64_t lda, int64_t m, double *v, int64_t ldv, i
         {
             return;
         }
-        const jsvd::RoundShape shape = jsvd::choose_round_shape(m, n, false);
+        const jsvd::RoundShape shape = jsvd::kernel_round_shape(m, n);
         const hipStream_t s = static_cast<hipStream_t>(stream);
         for (int32_t k = 0; k < rounds; ++k)
         {

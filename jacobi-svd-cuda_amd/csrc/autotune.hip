@@ -1,0 +1,314 @@
+// autotune.hip -- measured choice of launch parameters per matrix shape.
+//
+// For an m x n problem the tuner times, on synthetic data of that shape on the device:
+//   * every instantiated launch shape of the plain path's fused round kernel that suits the shape (64, 256 or
+//     1024 threads per column pair; columns held in registers with the smallest chunk count that fits, V
+//     held or streamed, or both streamed), up to 16 back-to-back rounds of a first sweep each;
+//   * the total workgroup count of the block paths' apply kernel (256 / 512 / 1024 per launch), up to 8 FP32
+//     block rounds each.
+// Every candidate starts from the same seeded data; the best of `repetitions` timings counts.  The fastest
+// choices are cached per (m, n) for the life of the process, and plans created or reshaped afterwards for
+// that shape use them (unless their layout_transpose_mode forces a launch shape).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "autotune.hpp"
+#include "block.hpp"
+
+namespace jsvd
+{
+    namespace
+    {
+        std::mutex g_mu;
+        std::map<std::pair<int64_t, int64_t>, Tuning> g_table;
+
+        __global__ void fill_kernel(double *x, int64_t count, uint64_t seed)
+        {
+            for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < count;
+                 i += int64_t{gridDim.x} * blockDim.x)
+            {
+                // splitmix64 of (seed, i) mapped to [-1, 1)
+                uint64_t z = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                x[i] = static_cast<double>(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            }
+        }
+
+        __global__ void fill_f32_kernel(float *x, const double *src, int64_t count)
+        {
+            for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < count;
+                 i += int64_t{gridDim.x} * blockDim.x)
+            {
+                x[i] = static_cast<float>(src[i]);
+            }
+        }
+
+        struct DevBuf
+        {
+            void *p = nullptr;
+            explicit DevBuf(size_t bytes) { JSVD_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256))); }
+            ~DevBuf() { (void)hipFree(p); }
+            DevBuf(const DevBuf &) = delete;
+            DevBuf &operator=(const DevBuf &) = delete;
+        };
+
+        struct Events
+        {
+            hipEvent_t a = nullptr, b = nullptr;
+            Events()
+            {
+                JSVD_HIP_CHECK(hipEventCreate(&a));
+                JSVD_HIP_CHECK(hipEventCreate(&b));
+            }
+            ~Events()
+            {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+            double elapsed_us() const
+            {
+                float ms = 0.0f;
+                JSVD_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+                return 1000.0 * ms;
+            }
+        };
+
+        unsigned grid_for(int64_t count)
+        {
+            return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 4096)));
+        }
+
+        // Launch shapes the round kernel is instantiated for that suit an m x n problem.
+        std::vector<RoundShape> round_candidates(int64_t m, int64_t n)
+        {
+            const int64_t m2 = (m + 1) / 2, n2 = (n + 1) / 2;
+            std::vector<RoundShape> out;
+            for (int t : {64, 256, 1024})
+            {
+                RoundShape stream_shape;
+                stream_shape.threads = t;
+                stream_shape.ea = 0;
+                stream_shape.ev = 0;
+                out.push_back(stream_shape);
+                for (int ea : {1, 2, 4, 8})
+                {
+                    if (static_cast<int64_t>(ea) * t < m2 || (ea > 1 && static_cast<int64_t>(ea / 2) * t >= m2))
+                    {
+                        continue; // the smallest register chunk count that holds a column
+                    }
+                    for (int ev : {0, 1, 2, 4, 8})
+                    {
+                        if (ev > ea || (ea == 1 && ev != 1) || (ea == 2 && ev == 0) ||
+                            (t == 1024 && ea == 8 && ev > 2))
+                        {
+                            continue;
+                        }
+                        if (ev > 0 && (static_cast<int64_t>(ev) * t < n2 || (ev > 1 && (ev / 2) * t >= n2)))
+                        {
+                            continue;
+                        }
+                        RoundShape s;
+                        s.threads = t;
+                        s.ea = ea;
+                        s.ev = ev;
+                        out.push_back(s);
+                    }
+                }
+            }
+            return out;
+        }
+
+        double time_rounds(const RoundShape &shape, double *a, int64_t lda, int64_t m, double *v, int64_t ldv,
+                           int64_t n, int *slots, unsigned long long *off, int rounds, hipStream_t stream)
+        {
+            Events ev;
+            const int per_sweep = static_cast<int>(n + (n & 1) - 1);
+            launch_round(shape, a, lda, m, v, ldv, n, 0, 1e-15, slots, off, nullptr, stream, 64); // warm-up
+            JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
+            for (int r = 1; r <= rounds; ++r)
+            {
+                launch_round(shape, a, lda, m, v, ldv, n, r % per_sweep, 1e-15, slots, off, nullptr, stream, 64);
+            }
+            JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
+            JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
+            return ev.elapsed_us() / rounds;
+        }
+    } // namespace
+
+    bool lookup_tuning(int64_t m, int64_t n, Tuning &out)
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        const auto it = g_table.find({m, n});
+        if (it == g_table.end())
+        {
+            return false;
+        }
+        out = it->second;
+        return true;
+    }
+
+    Tuning run_autotune(int64_t m, int64_t n, int repetitions, hipStream_t stream, jsvd_tune_report *report)
+    {
+        validate_shape(m, n);
+        if (repetitions < 1)
+        {
+            throw_invalid("repetitions must be positive.");
+        }
+        Tuning best;
+        best.round = choose_round_shape(m, n, false);
+        jsvd_tune_report rep{};
+        const int64_t lda = m + (m & 1), ldv = n + (n & 1);
+        // plain path: round-kernel launch shapes
+        if (n >= 2)
+        {
+            DevBuf a0(sizeof(double) * lda * n), a(sizeof(double) * lda * n), v(sizeof(double) * ldv * n);
+            DevBuf slots(sizeof(int) * 64), off(sizeof(unsigned long long) * 64);
+            hipLaunchKernelGGL(fill_kernel, dim3(grid_for(lda * n)), dim3(256), 0, stream, static_cast<double *>(a0.p),
+                               lda * n, uint64_t{12345});
+            JSVD_HIP_CHECK(hipGetLastError());
+            JSVD_HIP_CHECK(hipMemsetAsync(slots.p, 0, sizeof(int) * 64, stream));
+            JSVD_HIP_CHECK(hipMemsetAsync(off.p, 0, sizeof(unsigned long long) * 64, stream));
+            const int rounds = 16;
+            double best_us = -1.0;
+            for (const RoundShape &cand : round_candidates(m, n))
+            {
+                // same data for every candidate: A from the seeded fill (padding rows zero), V = I
+                // (the kernel reads the padding row of an odd m as zero whatever it holds)
+                double us = -1.0;
+                for (int rep = 0; rep < repetitions; ++rep)
+                {
+                    JSVD_HIP_CHECK(
+                        hipMemcpyAsync(a.p, a0.p, sizeof(double) * lda * n, hipMemcpyDeviceToDevice, stream));
+                    launch_identity(static_cast<double *>(v.p), ldv, n, stream);
+                    const double t = time_rounds(cand, static_cast<double *>(a.p), lda, m, static_cast<double *>(v.p),
+                                                 ldv, n, static_cast<int *>(slots.p),
+                                                 static_cast<unsigned long long *>(off.p), rounds, stream);
+                    us = (us < 0.0) ? t : std::min(us, t);
+                }
+                ++rep.candidates_tried;
+                const RoundShape def = choose_round_shape(m, n, false);
+                if (cand.threads == def.threads && cand.ea == def.ea && cand.ev == def.ev)
+                {
+                    rep.round_us_default = us;
+                }
+                if (best_us < 0.0 || us < best_us)
+                {
+                    best_us = us;
+                    best.round = cand;
+                }
+            }
+            rep.round_us_best = best_us;
+        }
+        // block paths: apply grid size (FP32 rounds)
+        const BlockGeometry g0 = BlockGeometry::make(m, n);
+        if (g0.nb >= 2)
+        {
+            BlockGeometry g = g0;
+            const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad, v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
+            DevBuf src(sizeof(double) * a_elems), a32(sizeof(float) * a_elems), v32(sizeof(float) * v_elems);
+            DevBuf part(sizeof(float) * g.partial_elems()), q(sizeof(float) * g.pairs() * kW * kW);
+            DevBuf flags(sizeof(int) * g.pairs()), perm(sizeof(int) * g.n_pad), norm2(sizeof(double) * g.n_pad);
+            hipLaunchKernelGGL(fill_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream, static_cast<double *>(src.p),
+                               static_cast<int64_t>(a_elems), uint64_t{777});
+            JSVD_HIP_CHECK(hipGetLastError());
+            launch_iota(static_cast<int *>(perm.p), g.n_pad, stream);
+            const int rounds = std::min(8, g.rounds());
+            double best_us = -1.0;
+            for (int wgs : {256, 512, 1024})
+            {
+                g.apply_wgs = wgs;
+                double us = -1.0;
+                for (int rep = 0; rep < repetitions; ++rep)
+                {
+                    hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream,
+                                       static_cast<float *>(a32.p), static_cast<const double *>(src.p),
+                                       static_cast<int64_t>(a_elems));
+                    JSVD_HIP_CHECK(hipGetLastError());
+                    launch_identity_f32(static_cast<float *>(v32.p), g.ldv, g.n_pad, stream);
+                    Events ev;
+                    JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        launch_block_round<float>(g, static_cast<float *>(a32.p), static_cast<float *>(v32.p), r,
+                                                  1e-7, static_cast<float *>(part.p), static_cast<float *>(q.p),
+                                                  static_cast<int *>(flags.p), nullptr, nullptr,
+                                                  static_cast<const int *>(perm.p), nullptr, nullptr, stream);
+                    }
+                    JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
+                    JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
+                    const double t = ev.elapsed_us() / rounds;
+                    us = (us < 0.0) ? t : std::min(us, t);
+                }
+                ++rep.candidates_tried;
+                if (wgs == 512)
+                {
+                    rep.block_round_us_default = us;
+                }
+                if (best_us < 0.0 || us < best_us)
+                {
+                    best_us = us;
+                    best.apply_wgs = wgs;
+                }
+            }
+            rep.block_round_us_best = best_us;
+        }
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream));
+        rep.round_threads = best.round.threads;
+        rep.round_ea = best.round.ea;
+        rep.round_ev = best.round.ev;
+        rep.apply_workgroups = best.apply_wgs;
+        {
+            std::lock_guard<std::mutex> lock(g_mu);
+            g_table[{m, n}] = best;
+        }
+        if (report)
+        {
+            *report = rep;
+        }
+        return best;
+    }
+} // namespace jsvd
+
+extern "C" int jsvd_autotune(int64_t m, int64_t n, int32_t repetitions, void *stream, jsvd_tune_report *report)
+{
+    return jsvd::guarded([&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        bool own = false;
+        if (!s)
+        {
+            JSVD_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            own = true;
+        }
+        try
+        {
+            (void)jsvd::run_autotune(m, n, repetitions, s, report);
+        }
+        catch (...)
+        {
+            if (own)
+            {
+                (void)hipStreamDestroy(s);
+            }
+            throw;
+        }
+        if (own)
+        {
+            JSVD_HIP_CHECK(hipStreamDestroy(s));
+        }
+    });
+}
+
+extern "C" int jsvd_autotune_clear(void)
+{
+    return jsvd::guarded([&] {
+        std::lock_guard<std::mutex> lock(jsvd::g_mu);
+        jsvd::g_table.clear();
+    });
+}

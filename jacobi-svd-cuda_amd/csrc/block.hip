@@ -870,10 +870,10 @@ namespace jsvd
                            static_cast<const T *>(partial), parts, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
                            offmax, perm, norm2, skip_if_zero);
         JSVD_HIP_CHECK(hipGetLastError());
-        // lda and ldv are multiples of kH, hence of the apply slab height; about two workgroups per CU in total
+        // lda and ldv are multiples of kH, hence of the apply slab height; g.apply_wgs workgroups in total
         const int h = apply_rows<T>();
         const int sa = static_cast<int>(g.lda / h), sv = static_cast<int>(g.ldv / h);
-        const int per_pair = std::max(1, std::min(sa + sv, (512 + pairs - 1) / pairs));
+        const int per_pair = std::max(1, std::min(sa + sv, (g.apply_wgs + pairs - 1) / pairs));
         hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, per_pair), dim3(256), 0, stream, a, g.lda, sa, v, g.ldv,
                            sv, g.nb, g.Nb, round, static_cast<const T *>(q), static_cast<const int *>(apply_flag),
                            perm, skip_if_zero);

@@ -22,6 +22,7 @@ namespace jsvd
         int64_t ldv = 0;    // rows of the padded V (multiple of kH, >= n_pad)
         int slabs_a = 0;    // lda / kH
         int slabs_v = 0;    // ldv / kH
+        int apply_wgs = 512; // target workgroups of one apply launch (two per CU); the autotuner may change it
 
         static BlockGeometry make(int64_t m, int64_t n)
         {

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 
+#include "autotune.hpp"
 #include "plain_f64.hpp"
 
 namespace jsvd
@@ -34,6 +35,15 @@ namespace jsvd
         double default_fp32_tolerance(int64_t m)
         {
             return std::max(1e-6, 4.0 * std::sqrt(static_cast<double>(m)) * std::ldexp(1.0, -24));
+        }
+        // The autotuner's apply grid size for this shape, when it measured one.
+        void apply_tuning(BlockGeometry &g)
+        {
+            Tuning t;
+            if (lookup_tuning(g.m, g.n, t))
+            {
+                g.apply_wgs = t.apply_wgs;
+            }
         }
     } // namespace
 
@@ -201,7 +211,7 @@ namespace jsvd
 
     bool BlockSolver::reshape(int64_t m, int64_t n)
     {
-        const BlockGeometry g = BlockGeometry::make(m, n);
+        BlockGeometry g = BlockGeometry::make(m, n);
         const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
         const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
         const size_t part_elems = g.partial_elems();
@@ -210,6 +220,7 @@ namespace jsvd
         {
             return false;
         }
+        apply_tuning(g);
         if (g.m != g_.m || g.n != g_.n)
         {
             // padding rows and columns of the new extent must read as zero

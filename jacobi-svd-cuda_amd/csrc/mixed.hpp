@@ -27,6 +27,7 @@ namespace jsvd
         // shape); false (nothing changed) when it does not fit or n < 33.
         bool reshape(int64_t m, int64_t n);
         void set_epsilon(double eps) { cfg_.epsilon = eps; }
+        [[nodiscard]] const BlockGeometry &geometry() const { return g_; }
 
     private:
         template <typename T>

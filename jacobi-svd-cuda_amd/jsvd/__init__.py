@@ -293,6 +293,14 @@ class Plan:
         check(lib().jsvd_plan_last_info(self._h, ctypes.byref(info)))
         return info
 
+    def launch_params(self) -> dict:
+        """The plan's current launch parameters (jsvd_plan_launch_params)."""
+        L = lib()
+        L.jsvd_plan_launch_params.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        vals = [ctypes.c_int32() for _ in range(4)]
+        check(L.jsvd_plan_launch_params(self._h, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(("round_threads", "round_ea", "round_ev", "apply_workgroups"), (v.value for v in vals)))
+
     def reshape(self, m: int, n: int, epsilon: float) -> None:
         """Re-targets the plan at m x n and epsilon inside its allocated capacity (jsvd_plan_reshape)."""
         L = lib()
@@ -687,3 +695,39 @@ def mat_stream_solve(input_path: str, output_path: Optional[str], config: Option
     check(L.jsvd_mat_stream_solve(input_path.encode(), output_path.encode() if output_path else None,
                                   ctypes.byref(cfg), ctypes.byref(mx), ctypes.byref(rep)))
     return rep
+
+
+class TuneReport(ctypes.Structure):
+    """jsvd_tune_report (include/jsvd.h)."""
+    _fields_ = [
+        ("candidates_tried", ctypes.c_int32),
+        ("round_threads", ctypes.c_int32),
+        ("round_ea", ctypes.c_int32),
+        ("round_ev", ctypes.c_int32),
+        ("apply_workgroups", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("round_us_default", ctypes.c_double),
+        ("round_us_best", ctypes.c_double),
+        ("block_round_us_default", ctypes.c_double),
+        ("block_round_us_best", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+def autotune(m: int, n: int, repetitions: int = 2, stream=None) -> TuneReport:
+    """Measures the launch parameters for m x n on the device and caches the fastest for plans of that shape
+    created or reshaped afterwards in this process (round-kernel launch shape, block apply grid size)."""
+    L = lib()
+    L.jsvd_autotune.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                ctypes.POINTER(TuneReport)]
+    rep = TuneReport()
+    s = _stream_handle(stream) if stream is not None else None
+    check(L.jsvd_autotune(m, n, repetitions, s, ctypes.byref(rep)))
+    return rep
+
+
+def autotune_clear() -> None:
+    """Forgets every cached autotune result."""
+    check(lib().jsvd_autotune_clear())
