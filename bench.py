@@ -227,14 +227,16 @@ def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
     }
 
 
-def time_path(torch, jsvd, path, a_host, sigma_true, cfg, steps, warmup):
-    """Warmup + `steps` timed device-resident solves through one plan; returns the report of the path."""
+def time_path(torch, jsvd, path, a_host, sigma_true, cfg, steps, warmup, block_width=0):
+    """Warmup + `steps` timed device-resident solves through one plan; returns the report of the path.
+    block_width 0 takes the autotuner's width (or the default without tuning)."""
     m, n = a_host.shape
     d_a = torch.from_numpy(a_host).to("cuda")
     d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
     d_s = torch.empty(n, dtype=torch.float64, device="cuda")
     d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
-    plan = jsvd.Plan(m, n, cfg, path=path)
+    plan = jsvd.Plan(m, n, cfg, path=path, mixed=jsvd.mixed_config(path, block_width=block_width))
+    launch = plan.launch_params()
     for _ in range(warmup):
         plan.solve_device(d_a, d_u, d_s, d_v)
     torch.cuda.synchronize()
@@ -268,6 +270,7 @@ def time_path(torch, jsvd, path, a_host, sigma_true, cfg, steps, warmup):
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "info": info,
+        "launch": launch,
         "accuracy": acc,
         "sigma": sig,
     }
@@ -283,6 +286,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
     ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
     ap.add_argument("--no-file", action="store_true", help="skip the streamed *.mat file leg")
+    ap.add_argument("--block-width", type=int, default=0, choices=[0, 16, 32, 64],
+                    help="block width of the mixed path (0: the autotuner's choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch parameters (no jsvd.autotune before the timed legs)")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving leg")
@@ -326,7 +331,8 @@ def main() -> None:
         return
 
     log(f"[bench] mixed path: warmup {args.warmup}, {args.steps} timed steps")
-    mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, args.steps, args.warmup)
+    mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, args.steps, args.warmup,
+                      block_width=args.block_width)
     log(f"[bench] mixed: {mixed['solves_per_s']:.2f} solves/s info={mixed['info']} acc={mixed['accuracy']}")
     plain = None
     if not args.no_plain:

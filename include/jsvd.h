@@ -110,7 +110,7 @@ int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *off
  * rule "stop after a sweep without rotations", applied to block sweeps).
  * ------------------------------------------------------------------------------------------------- */
 #define JSVD_PATH_PLAIN_F64 0 /* column-pair cyclic Jacobi, the reference algorithm */
-#define JSVD_PATH_BLOCK_F64 1 /* block Jacobi (32-column blocks), FP64 MFMA Gram and rotate */
+#define JSVD_PATH_BLOCK_F64 1 /* block Jacobi (16/32/64-column blocks), FP64 MFMA Gram and rotate */
 #define JSVD_PATH_MIXED 2     /* FP32 MFMA block sweeps, FP64 re-orthogonalisation, FP64 block finishing */
 
 typedef struct jsvd_mixed_config
@@ -119,7 +119,8 @@ typedef struct jsvd_mixed_config
     int32_t fp32_max_sweeps;     /* cap of the FP32 block sweeps (default 40) */
     double fp32_tolerance;       /* FP32 phase threshold; <= 0 selects max(1e-6, 4 sqrt(m) 2^-24) */
     int32_t orthogonalize_steps; /* Newton-Schulz steps applied to the FP32 V in FP64 (default 2) */
-    int32_t reserved;
+    int32_t block_width;         /* columns per block: 16, 32 or 64; 0 (default) = jsvd_autotune's choice for the
+                                    shape when it measured one, else 32; halved while n <= width */
 } jsvd_mixed_config;
 
 /* Per-solve report of a plan. */
@@ -157,12 +158,13 @@ int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon);
 
 /*
  * The launch parameters a plan currently uses: for the plain path the fused round kernel's threads per column
- * pair and register chunk counts (0 = streamed) with apply_workgroups 0; for the block paths the apply
- * kernel's total workgroups with the round fields 0.  They come from jsvd_autotune's cache when it measured
- * the plan's shape, else from the built-in rules.  No reference counterpart.
+ * pair and register chunk counts (0 = streamed), block fields 0; for the block paths the apply kernel's total
+ * workgroups and the block width, round fields 0.  They come from the mixed config's block_width, from
+ * jsvd_autotune's cache when it measured the plan's shape, else from the built-in rules.  No reference
+ * counterpart.
  */
 int jsvd_plan_launch_params(const jsvd_plan *plan, int32_t *round_threads, int32_t *round_ea, int32_t *round_ev,
-                            int32_t *apply_workgroups);
+                            int32_t *apply_workgroups, int32_t *block_width);
 
 /* One-shot host-pointer solve through the path selected by `mixed` (NULL: plain FP64). */
 int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
@@ -327,18 +329,21 @@ typedef struct jsvd_tune_report
     int32_t round_ea;              /* plain path: register chunks per thread for an A column (0 = streamed) */
     int32_t round_ev;              /* plain path: register chunks per thread for a V column (0 = streamed) */
     int32_t apply_workgroups;      /* block paths: total workgroups of one apply launch */
-    int32_t reserved;
+    int32_t block_width;           /* block paths: chosen block width (16, 32 or 64; 0 if n <= 16) */
     double round_us_default;       /* us per round of the shape chosen by the built-in rule */
     double round_us_best;          /* us per round of the chosen shape */
-    double block_round_us_default; /* us per FP32 block round with 512 apply workgroups (0 if n <= 32) */
-    double block_round_us_best;    /* us per FP32 block round with the chosen count (0 if n <= 32) */
+    double block_sweep_us[3];      /* estimated us per FP32 block sweep at widths 16, 32, 64 (512 apply
+                                      workgroups; 0 where the width leaves fewer than two blocks) */
+    double block_round_us_default; /* us per FP32 block round at the chosen width, 512 apply workgroups */
+    double block_round_us_best;    /* us per FP32 block round at the chosen width and workgroup count */
 } jsvd_tune_report;
 
 /*
  * Autotuner of the launch parameters for m x n (m >= n >= 1): times on the device, on seeded data of that
- * shape, every round-kernel launch shape that suits it (16 rounds of a first sweep each) and the block apply
- * kernel's grid size (256 / 512 / 1024 workgroups, 8 FP32 block rounds each), keeping the best of
- * `repetitions` timings per candidate.  The winners are cached for the process: plans created or reshaped
+ * shape, every round-kernel launch shape that suits it (16 rounds of a first sweep each), the block width
+ * (16 / 32 / 64 columns: up to 8 FP32 block rounds each, compared per sweep, i.e. per round times the rounds a
+ * sweep takes at that width) and, at the chosen width, the block apply kernel's grid size (256 / 512 / 1024
+ * workgroups), keeping the best of `repetitions` timings per candidate.  The winners are cached for the process: plans created or reshaped
  * to m x n afterwards use them, except that layout_transpose_mode FORCE_ENABLE / FORCE_DISABLE still fix the
  * round kernel's shape.  stream may be NULL (a private stream).  No counterpart in the reference, whose only
  * tuning is the layout-transpose threshold search (src/kernels.cu:1078-1127; auto_tune_layout_transpose_threshold
@@ -358,16 +363,16 @@ int jsvd_autotune_clear(void);
 int jsvd_round_robin_pairs(int64_t n, int32_t round, int32_t *pairs_out);
 
 /*
- * The inner step of a block round alone (block and mixed paths): for each of `pairs` independent 64 x 64
- * Gram matrices gram[p] (row-major, device, upper triangle read; float when precision is
- * JSVD_PRECISION_F32, double when JSVD_PRECISION_F64), one cyclic Jacobi sweep in Gram space in the
- * round-robin order of 64 columns, rotation parameters in FP64 with the reference's skip test
- * (src/kernels.cu:503-511 applied to the Gram entries).  q[p] (row-major 64 x 64, same type) receives the
+ * The inner step of a block round alone (block and mixed paths): for each of `pairs` independent kw x kw
+ * Gram matrices gram[p] (kw = 2 block_width, block_width 16, 32 or 64; row-major, device, upper triangle read;
+ * float when precision is JSVD_PRECISION_F32, double when JSVD_PRECISION_F64), one cyclic Jacobi sweep in Gram
+ * space in the round-robin order of kw columns, rotation parameters in FP64 with the reference's skip test
+ * (src/kernels.cu:503-511 applied to the Gram entries).  q[p] (row-major kw x kw, same type) receives the
  * accumulated rotation Q (G <- Q^T G Q), rotations[p] (device int32) the number of rotated column pairs.
  * No reference counterpart: the reference has no block path.
  */
-int jsvd_block_eigen(int32_t precision, const void *gram, int64_t pairs, double epsilon, void *q,
-                     int32_t *rotations, void *stream);
+int jsvd_block_eigen(int32_t precision, int32_t block_width, const void *gram, int64_t pairs, double epsilon,
+                     void *q, int32_t *rotations, void *stream);
 
 /* ---------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, enqueued on `stream`, asynchronous).  They expose the

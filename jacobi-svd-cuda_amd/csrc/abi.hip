@@ -576,6 +576,10 @@ namespace
         {
             jsvd::throw_invalid("unknown solver path.");
         }
+        if (mx.block_width != 0 && !jsvd::valid_block_width(mx.block_width))
+        {
+            jsvd::throw_invalid("block_width must be 0, 16, 32 or 64.");
+        }
         if (mx.path == JSVD_PATH_MIXED && (mx.fp32_max_sweeps <= 0 || mx.orthogonalize_steps < 1 ||
                                            mx.orthogonalize_steps > 8 || !(mx.fp32_tolerance < 1.0)))
         {
@@ -636,16 +640,17 @@ int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon)
 }
 
 int jsvd_plan_launch_params(const jsvd_plan *plan, int32_t *round_threads, int32_t *round_ea, int32_t *round_ev,
-                            int32_t *apply_workgroups)
+                            int32_t *apply_workgroups, int32_t *block_width)
 {
     return jsvd::guarded([&] {
-        require(plan != nullptr && round_threads && round_ea && round_ev && apply_workgroups,
+        require(plan != nullptr && round_threads && round_ea && round_ev && apply_workgroups && block_width,
                 "plan and outputs must not be null.");
         const bool blk = plan->block != nullptr;
         *round_threads = blk ? 0 : plan->shape.threads;
         *round_ea = blk ? 0 : plan->shape.ea;
         *round_ev = blk ? 0 : plan->shape.ev;
         *apply_workgroups = blk ? plan->block->geometry().apply_wgs : 0;
+        *block_width = blk ? plan->block->geometry().kb : 0;
     });
 }
 
@@ -672,11 +677,12 @@ int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg
     });
 }
 
-int jsvd_block_eigen(int32_t precision, const void *gram, int64_t pairs, double epsilon, void *q,
-                     int32_t *rotations, void *stream)
+int jsvd_block_eigen(int32_t precision, int32_t block_width, const void *gram, int64_t pairs, double epsilon,
+                     void *q, int32_t *rotations, void *stream)
 {
     return jsvd::guarded([&] {
         require(precision == JSVD_PRECISION_F32 || precision == JSVD_PRECISION_F64, "unknown precision.");
+        require(jsvd::valid_block_width(block_width), "block_width must be 16, 32 or 64.");
         require(pairs >= 0 && pairs <= (int64_t{1} << 20), "pairs out of range.");
         require(epsilon > 0.0, "epsilon must be positive.");
         if (pairs == 0)
@@ -687,13 +693,13 @@ int jsvd_block_eigen(int32_t precision, const void *gram, int64_t pairs, double 
         const hipStream_t s = static_cast<hipStream_t>(stream);
         if (precision == JSVD_PRECISION_F32)
         {
-            jsvd::launch_block_eigen(static_cast<const float *>(gram), static_cast<int>(pairs), epsilon,
+            jsvd::launch_block_eigen(static_cast<const float *>(gram), block_width, static_cast<int>(pairs), epsilon,
                                      static_cast<float *>(q), rotations, s);
         }
         else
         {
-            jsvd::launch_block_eigen(static_cast<const double *>(gram), static_cast<int>(pairs), epsilon,
-                                     static_cast<double *>(q), rotations, s);
+            jsvd::launch_block_eigen(static_cast<const double *>(gram), block_width, static_cast<int>(pairs),
+                                     epsilon, static_cast<double *>(q), rotations, s);
         }
     });
 }

@@ -4,8 +4,9 @@
 //   * every instantiated launch shape of the plain path's fused round kernel that suits the shape (64, 256 or
 //     1024 threads per column pair; columns held in registers with the smallest chunk count that fits, V
 //     held or streamed, or both streamed), up to 16 back-to-back rounds of a first sweep each;
-//   * the total workgroup count of the block paths' apply kernel (256 / 512 / 1024 per launch), up to 8 FP32
-//     block rounds each.
+//   * the block paths' block width (16, 32, 64 columns), up to 8 FP32 block rounds each, compared per sweep
+//     (the round time times the rounds of a sweep at that width), then at the chosen width the apply kernel's
+//     total workgroup count (256 / 512 / 1024 per launch).
 // Every candidate starts from the same seeded data; the best of `repetitions` timings counts.  The fastest
 // choices are cached per (m, n) for the life of the process, and plans created or reshaped afterwards for
 // that shape use them (unless their layout_transpose_mode forces a launch shape).
@@ -140,6 +141,45 @@ namespace jsvd
             JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
             return ev.elapsed_us() / rounds;
         }
+
+        // Microseconds per FP32 block round of g on seeded data: up to 8 rounds of a first sweep, best of
+        // `repetitions`.
+        double time_block_rounds(const BlockGeometry &g, int repetitions, hipStream_t stream)
+        {
+            const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
+            const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
+            DevBuf src(sizeof(double) * a_elems), a32(sizeof(float) * a_elems), v32(sizeof(float) * v_elems);
+            DevBuf part(sizeof(float) * g.partial_elems()), q(sizeof(float) * g.q_elems());
+            DevBuf flags(sizeof(int) * g.pairs()), perm(sizeof(int) * g.n_pad);
+            hipLaunchKernelGGL(fill_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream, static_cast<double *>(src.p),
+                               static_cast<int64_t>(a_elems), uint64_t{777});
+            JSVD_HIP_CHECK(hipGetLastError());
+            launch_iota(static_cast<int *>(perm.p), g.n_pad, stream);
+            const int rounds = std::min(8, g.rounds());
+            double us = -1.0;
+            for (int rep = 0; rep < repetitions; ++rep)
+            {
+                hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream,
+                                   static_cast<float *>(a32.p), static_cast<const double *>(src.p),
+                                   static_cast<int64_t>(a_elems));
+                JSVD_HIP_CHECK(hipGetLastError());
+                launch_identity_f32(static_cast<float *>(v32.p), g.ldv, g.n_pad, stream);
+                Events ev;
+                JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
+                for (int r = 0; r < rounds; ++r)
+                {
+                    launch_block_round<float>(g, static_cast<float *>(a32.p), static_cast<float *>(v32.p), r, 1e-7,
+                                              static_cast<float *>(part.p), static_cast<float *>(q.p),
+                                              static_cast<int *>(flags.p), nullptr, nullptr,
+                                              static_cast<const int *>(perm.p), nullptr, nullptr, stream);
+                }
+                JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
+                JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
+                const double t = ev.elapsed_us() / rounds;
+                us = (us < 0.0) ? t : std::min(us, t);
+            }
+            return us;
+        }
     } // namespace
 
     bool lookup_tuning(int64_t m, int64_t n, Tuning &out)
@@ -206,58 +246,44 @@ namespace jsvd
             }
             rep.round_us_best = best_us;
         }
-        // block paths: apply grid size (FP32 rounds)
-        const BlockGeometry g0 = BlockGeometry::make(m, n);
-        if (g0.nb >= 2)
+        // block paths: the block width (per-sweep time), then the apply grid size at that width (FP32 rounds)
+        const int widths[3] = {16, 32, 64};
+        double best_sweep = -1.0;
+        for (int wi = 0; wi < 3; ++wi)
         {
-            BlockGeometry g = g0;
-            const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad, v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
-            DevBuf src(sizeof(double) * a_elems), a32(sizeof(float) * a_elems), v32(sizeof(float) * v_elems);
-            DevBuf part(sizeof(float) * g.partial_elems()), q(sizeof(float) * g.pairs() * kW * kW);
-            DevBuf flags(sizeof(int) * g.pairs()), perm(sizeof(int) * g.n_pad), norm2(sizeof(double) * g.n_pad);
-            hipLaunchKernelGGL(fill_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream, static_cast<double *>(src.p),
-                               static_cast<int64_t>(a_elems), uint64_t{777});
-            JSVD_HIP_CHECK(hipGetLastError());
-            launch_iota(static_cast<int *>(perm.p), g.n_pad, stream);
-            const int rounds = std::min(8, g.rounds());
-            double best_us = -1.0;
-            for (int wgs : {256, 512, 1024})
+            const BlockGeometry g = BlockGeometry::make(m, n, widths[wi]);
+            if (g.nb < 2)
+            {
+                continue;
+            }
+            const double us = time_block_rounds(g, repetitions, stream);
+            ++rep.candidates_tried;
+            rep.block_sweep_us[wi] = us * g.rounds();
+            if (best_sweep < 0.0 || rep.block_sweep_us[wi] < best_sweep)
+            {
+                best_sweep = rep.block_sweep_us[wi];
+                best.block_width = widths[wi];
+                rep.block_round_us_default = us;
+            }
+        }
+        if (best_sweep >= 0.0)
+        {
+            BlockGeometry g = BlockGeometry::make(m, n, best.block_width);
+            double best_us = rep.block_round_us_default; // 512 workgroups, measured above
+            best.apply_wgs = g.apply_wgs;
+            for (int wgs : {256, 1024})
             {
                 g.apply_wgs = wgs;
-                double us = -1.0;
-                for (int rep = 0; rep < repetitions; ++rep)
-                {
-                    hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream,
-                                       static_cast<float *>(a32.p), static_cast<const double *>(src.p),
-                                       static_cast<int64_t>(a_elems));
-                    JSVD_HIP_CHECK(hipGetLastError());
-                    launch_identity_f32(static_cast<float *>(v32.p), g.ldv, g.n_pad, stream);
-                    Events ev;
-                    JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
-                    for (int r = 0; r < rounds; ++r)
-                    {
-                        launch_block_round<float>(g, static_cast<float *>(a32.p), static_cast<float *>(v32.p), r,
-                                                  1e-7, static_cast<float *>(part.p), static_cast<float *>(q.p),
-                                                  static_cast<int *>(flags.p), nullptr, nullptr,
-                                                  static_cast<const int *>(perm.p), nullptr, nullptr, stream);
-                    }
-                    JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
-                    JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
-                    const double t = ev.elapsed_us() / rounds;
-                    us = (us < 0.0) ? t : std::min(us, t);
-                }
+                const double us = time_block_rounds(g, repetitions, stream);
                 ++rep.candidates_tried;
-                if (wgs == 512)
-                {
-                    rep.block_round_us_default = us;
-                }
-                if (best_us < 0.0 || us < best_us)
+                if (us < best_us)
                 {
                     best_us = us;
                     best.apply_wgs = wgs;
                 }
             }
             rep.block_round_us_best = best_us;
+            rep.block_width = best.block_width;
         }
         JSVD_HIP_CHECK(hipStreamSynchronize(stream));
         rep.round_threads = best.round.threads;

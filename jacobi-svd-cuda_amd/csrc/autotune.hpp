@@ -1,6 +1,7 @@
 // autotune.hpp -- measured launch parameters per matrix shape (autotune.hip).
 #pragma once
 
+#include "block.hpp"
 #include "plain_f64.hpp"
 
 namespace jsvd
@@ -9,6 +10,7 @@ namespace jsvd
     {
         RoundShape round{};  // plain path: round-kernel launch shape
         int apply_wgs = 512; // block paths: total workgroups of one apply launch
+        int block_width = 32; // block paths: columns per block
     };
 
     // The cached choice for m x n, if jsvd_autotune measured it in this process.

@@ -1,17 +1,18 @@
 // block.hip -- block one-sided Jacobi kernels (FP32 and FP64) for the mixed-precision path.
 //
-// Columns are grouped into blocks of kB = 32; a round pairs blocks (I, J) by the same round-robin circle
-// the plain path uses for columns.  For every block pair, one round is:
+// Columns are grouped into blocks of kb = 16, 32 or 64 (a template parameter of every kernel; the autotuner or
+// the caller picks it per shape); a round pairs blocks (I, J) by the same round-robin circle the plain path uses
+// for columns.  For every block pair (2 kb = kw columns), one round is:
 //
-//   block_gram_kernel    G_s = X_s^T X_s for each row slab X_s (kH rows x kW = 64 columns of [A_I A_J]) on
-//                        MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64), accumulating in T
-//   block_eigen_kernel   G = sum_s G_s, then one cyclic Jacobi sweep over the 64 columns carried out in Gram
+//   block_gram_kernel    G_s = X_s^T X_s for each row slab X_s (kH rows x kw columns of [A_I A_J]) on MFMA
+//                        (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64), accumulating in T
+//   block_eigen_kernel   G = sum_s G_s, then one cyclic Jacobi sweep over the kw columns carried out in Gram
 //                        space (two-sided 2x2 updates of G), rotation parameters (tau, t, c, s) in FP64 with
-//                        the reference skip test; accumulates the 64 x 64 rotation Q
+//                        the reference skip test; accumulates the kw x kw rotation Q
 //   block_apply_kernel   [A_I A_J] <- [A_I A_J] Q and [V_I V_J] <- [V_I V_J] Q per row slab on MFMA
 //
 // Layout: X (A or V) column-major with leading dimension ld (a multiple of kH, padding rows zero); the column
-// count is padded to a multiple of kB with zero columns (their Gram entries are zero, so they never rotate).
+// count is padded to a multiple of kb with zero columns (their Gram entries are zero, so they never rotate).
 #include "block.hpp"
 
 namespace jsvd
@@ -48,49 +49,60 @@ namespace jsvd
             J = dev::circle_at(Nb - 1 - slot, round, Nb);
         }
 
-        // Physical column of local column c (0..kW-1) of block pair (I, J) under the column map `perm`
+        // Physical column of local column c (0..2 KB - 1) of block pair (I, J) under the column map `perm`
         // (logical column k of the sweep order is stored in physical column perm[k]).
+        template <int KB>
         __device__ inline int64_t pair_column(const int *__restrict__ perm, int I, int J, int c)
         {
-            return perm[(c < kB) ? (I * kB + c) : (J * kB + c - kB)];
+            return perm[(c < KB) ? (I * KB + c) : (J * KB + c - KB)];
         }
 
-        // Loads rows [r0, r0+kH) of the kW columns [X_I X_J] into LDS as xs[c][r] (column-major, padded).
-        template <typename T>
-        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, const int *__restrict__ perm, int I,
-                                         int J, int64_t r0, T (*xs)[kH + 1])
+        // Rows of a kH slab staged in LDS per Gram step: all of them, or half at kb = 64 (LDS for 128 FP64
+        // columns, registers for the loads in flight).
+        template <typename T, int KB>
+        constexpr int gram_rows()
         {
-            // kW * kH elements, 256 threads: each thread moves kW*kH/256 elements; consecutive threads read
-            // consecutive rows of one column (coalesced).  All loads are issued before the LDS stores so
-            // they are in flight together.
-            constexpr int kPer = kW * kH / 256;
+            return (KB == 64) ? kH / 2 : kH;
+        }
+
+        // Loads rows [r0, r0+GH) of the 2 KB columns [X_I X_J] into LDS as xs[c][r] (column-major, padded).
+        template <typename T, int KB, int GH>
+        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, const int *__restrict__ perm, int I,
+                                         int J, int64_t r0, T (*xs)[GH + 1])
+        {
+            // 2 KB * GH elements, 256 threads; consecutive threads read consecutive rows of one column
+            // (coalesced).  All loads are issued before the LDS stores so they are in flight together.
+            constexpr int kPer = 2 * KB * GH / 256;
             T tmp[kPer];
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
             {
                 const int e = threadIdx.x + 256 * k;
-                const int c = e / kH;
-                const int r = e % kH;
-                tmp[k] = x[pair_column(perm, I, J, c) * ld + r0 + r];
+                tmp[k] = x[pair_column<KB>(perm, I, J, e / GH) * ld + r0 + e % GH];
             }
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
             {
                 const int e = threadIdx.x + 256 * k;
-                xs[e / kH][e % kH] = tmp[k];
+                xs[e / GH][e % GH] = tmp[k];
             }
         }
 
-        // Partial Gram of row slabs [ps * per, min(slabs, (ps + 1) * per)) of block pair `slot`; the partials of a
-        // pair are partial[slot][0 .. parts).
-        template <typename T>
+        // Partial Gram of kH-row slabs [ps * per, min(slabs, (ps + 1) * per)) of block pair `slot`; the partials
+        // of a pair are partial[slot][0 .. parts), each 2 KB x 2 KB row-major.  The (2 KB / 16)^2 output tiles are
+        // split evenly over the four waves.
+        template <typename T, int KB>
         __global__ __launch_bounds__(256) void block_gram_kernel(const T *__restrict__ a, int64_t lda,
                                                                  const int *__restrict__ perm, int nb, int Nb,
                                                                  int round, int slabs, int per, int parts,
                                                                  T *__restrict__ partial,
                                                                  const int *__restrict__ skip_if_zero)
         {
-            __shared__ T xs[kW][kH + 1];
+            constexpr int KW = 2 * KB;
+            constexpr int GH = gram_rows<T, KB>();
+            constexpr int NT = KW / 16;         // tiles per dimension
+            constexpr int TPW = NT * NT / 4;    // tiles per wave
+            __shared__ T xs[KW][GH + 1];
             if (skip_if_zero != nullptr && *skip_if_zero == 0)
             {
                 return;
@@ -105,100 +117,169 @@ namespace jsvd
             }
             const int lane = threadIdx.x % 64;
             const int wave = threadIdx.x / 64;
-            const int i0 = 16 * wave; // this wave computes G rows i0..i0+15, all kW columns
-            typename Mfma<T>::Acc acc[4];
-            for (int t = 0; t < 4; ++t)
+            // tile k of this wave: row tile rt(k), column tile ct(k) (k is a compile-time index after unrolling,
+            // so tiles that share a row share its operand load)
+            auto rt = [&](int k) { return (TPW >= NT) ? wave * (TPW / NT) + k / NT : (wave * TPW + k) / NT; };
+            auto ct = [&](int k) { return (TPW >= NT) ? k % NT : (wave * TPW + k) % NT; };
+            typename Mfma<T>::Acc acc[TPW];
+#pragma unroll
+            for (int k = 0; k < TPW; ++k)
             {
-                acc[t] = typename Mfma<T>::Acc{0, 0, 0, 0};
+                acc[k] = typename Mfma<T>::Acc{0, 0, 0, 0};
             }
             const int s_end = min(slabs, (ps + 1) * per);
+            bool first = true;
             for (int s = ps * per; s < s_end; ++s)
             {
-                if (s > ps * per)
+                for (int h = 0; h < kH; h += GH)
                 {
-                    __syncthreads(); // every wave is done with the previous slab
-                }
-                load_slab(a, lda, perm, I, J, static_cast<int64_t>(s) * kH, xs);
-                __syncthreads();
-                for (int r0 = 0; r0 < kH; r0 += 4)
-                {
-                    const int kr = r0 + lane / 16;
-                    const T av = xs[i0 + lane % 16][kr];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    if (!first)
                     {
-                        const T bv = xs[16 * t + lane % 16][kr];
-                        acc[t] = Mfma<T>::step(av, bv, acc[t]);
+                        __syncthreads(); // every wave is done with the previous rows
+                    }
+                    first = false;
+                    load_slab<T, KB, GH>(a, lda, perm, I, J, static_cast<int64_t>(s) * kH + h, xs);
+                    __syncthreads();
+                    for (int r0 = 0; r0 < GH; r0 += 4)
+                    {
+                        const int kr = r0 + lane / 16;
+#pragma unroll
+                        for (int k = 0; k < TPW; ++k)
+                        {
+                            const T av = xs[16 * rt(k) + lane % 16][kr];
+                            const T bv = xs[16 * ct(k) + lane % 16][kr];
+                            acc[k] = Mfma<T>::step(av, bv, acc[k]);
+                        }
                     }
                 }
             }
-            T *out = partial + (static_cast<int64_t>(slot) * parts + ps) * (kW * kW);
+            T *out = partial + (static_cast<int64_t>(slot) * parts + ps) * (KW * KW);
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int k = 0; k < TPW; ++k)
             {
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                 {
-                    const int i = i0 + Mfma<T>::row(lane, v);
-                    const int j = 16 * t + lane % 16;
-                    out[i * kW + j] = acc[t][v];
+                    const int i = 16 * rt(k) + Mfma<T>::row(lane, v);
+                    const int j = 16 * ct(k) + lane % 16;
+                    out[i * KW + j] = acc[k][v];
                 }
             }
         }
 
         // Reduction of the slab partials and one inner cyclic Jacobi sweep in Gram space.
         //
-        // G is symmetric and kept only in its upper triangle (g[min][max]), in two LDS buffers.  Each inner round
-        // pairs the 64 columns by the round-robin circle (pair u of a round = circle positions u and 63 - u).
-        // Inner round r is one phase and one barrier:
-        //   * the G waves (one 2x2 block (t, u), t <= u, of the 32 x 32 pair grid per thread) write
+        // G is symmetric and kept once per unordered entry, in two LDS buffers.  Each inner round pairs the kw
+        // columns by the round-robin circle (pair u of a round = circle positions u and kw - 1 - u).  Inner round
+        // r is one phase and one barrier:
+        //   * the G threads (2x2 blocks (t, u), t <= u, of the kb x kb pair grid; one or more per thread) write
         //     J_t^T G_r J_u into the other buffer, with the rotations of round r published in the previous phase;
-        //   * the Q waves apply the same rotations to Q;
-        //   * wave 0 meanwhile forms the rotations of round r+1 (lane l: pair l % 32, FP64, reference skip test)
+        //   * the Q threads apply the same rotations to Q;
+        //   * wave 0 meanwhile forms the rotations of round r+1 (lane l: pair l % kb, FP64, reference skip test)
         //     without waiting for G_{r+1}: the three entries a rotation needs are the two new diagonal entries of
         //     its columns and their new off-diagonal entry, each a 2x2 update of entries of G_r with rotations
         //     of round r that wave 0 already holds (exchanged between its lanes), evaluated by the same
-        //     expressions as the G waves use.
+        //     expressions as the G threads use.
         // So the FP64 rotation chain of the next round overlaps the bulk update of this one.  Vector-instruction
         // issue bounds the bulk update, so it is packed into as few waves as it needs.  Q is kept in registers in
-        // circle-position order: thread (row k, segment s) of the Q waves holds Q[k][col(p)] for the positions
-        // p = 8s..8s+7 and their partners 63-p, so every rotation of a round is local to a thread; after a round
-        // the positions move by one along the circle (1..63 cyclically, 0 fixed), which moves one value to each
-        // neighbouring segment.  Column indices advance incrementally (col(p, r+1) = col(p, r) - 1 on 1..63).  A
-        // block pair none of whose 2016 column pairs passes the skip test is left untouched.
-        constexpr int kEigenThreads = 1024;
-        constexpr int kPairs = kW / 2;
-        constexpr int kTriBlocks = kPairs * (kPairs + 1) / 2;   // 2x2 blocks of the upper triangle (528)
-        constexpr int kQThreads = 256;                          // 64 rows x 4 segments of 8 position pairs
-        constexpr int kQFirst = kEigenThreads - kQThreads;      // Q waves are the last four (12..15)
-        // Waves are spread round-robin over the CU's four SIMDs (wave w on SIMD w % 4).  Wave 0 forms the
-        // rotations, so the G blocks go to the waves 1..11 not on its SIMD (1, 2, 3, 5, 6, 7, 9, 10, 11): every
-        // SIMD then issues one Q wave and either the rotation wave or a third G wave.
-        constexpr int kGWaves = 9;
-        static_assert(kW == 64 && kGWaves * 64 >= kTriBlocks, "eigen thread roles assume 64 columns");
+        // circle-position order: thread (row k, segment s) of the Q threads holds Q[k][col(p)] for the QP positions
+        // p = QP s .. QP s + QP - 1 and their partners kw - 1 - p, so every rotation of a round is local to a
+        // thread; after a round the positions move by one along the circle (1..kw-1 cyclically, 0 fixed), which
+        // moves one value to each neighbouring segment.  Column indices advance incrementally (col(p, r+1) =
+        // col(p, r) - 1 on 1..kw-1).  A block pair none of whose column pairs passes the skip test is left
+        // untouched.
+        //
+        // Thread roles per block width (waves are spread round-robin over the CU's four SIMDs, wave w on SIMD
+        // w % 4):
+        //   kb = 16: 320 threads; wave 0 rotations, waves 1-3 G (136 blocks), wave 4 Q (32 rows x 2 segments).
+        //   kb = 32: 1024 threads; wave 0 rotations, G on the waves 1..11 not on its SIMD (1, 2, 3, 5, 6, 7, 9,
+        //            10, 11: 528 blocks) so every SIMD issues one Q wave and either the rotation wave or a third
+        //            G wave, waves 12-15 Q (64 rows x 4 segments of 8 position pairs).
+        //   kb = 64: 1024 threads; every thread holds Q (128 rows x 8 segments of 8 position pairs), wave 0 also
+        //            forms the rotations, waves 1-15 also update G (2080 blocks, up to 3 per thread) -- separate
+        //            Q waves would need more registers than a 1024-thread workgroup has.
+        template <int KB>
+        struct EigenShape;
+        template <>
+        struct EigenShape<16>
+        {
+            static constexpr int kThreads = 320, kQP = 8, kGThreads = 192;
+        };
+        template <>
+        struct EigenShape<32>
+        {
+            static constexpr int kThreads = 1024, kQP = 8, kGThreads = 576;
+        };
+        template <>
+        struct EigenShape<64>
+        {
+            static constexpr int kThreads = 1024, kQP = 8, kGThreads = 960;
+        };
+
+        // LDS image of the symmetric kw x kw G, one slot per unordered entry.  kb <= 32: the upper triangle of a
+        // square array with a padded row.  kb = 64 (two buffers of a square array would not fit for FP64):
+        // entry {i, j} at row i, offset d = (j - i) mod kw for d < kw / 2 (row min(i, j) for d = kw / 2), else at
+        // row j, offset kw - d -- kw (kw / 2 + 1) slots.
+        template <int KB>
+        struct GramLds
+        {
+            static constexpr int KW = 2 * KB;
+            static constexpr bool kCirc = KB == 64;
+            static constexpr int kStride = kCirc ? KW / 2 + 1 : KW + 1;
+            static constexpr int kElems = KW * kStride;
+            __device__ static int at(int i, int j)
+            {
+                if constexpr (kCirc)
+                {
+                    int d = (j - i) & (KW - 1);
+                    if (d > KW / 2 || (d == KW / 2 && i > j))
+                    {
+                        i = j;
+                        d = KW - d;
+                    }
+                    return i * kStride + d;
+                }
+                else
+                {
+                    return min(i, j) * kStride + max(i, j);
+                }
+            }
+        };
 
         // Index of this thread among the G threads, or -1.
+        template <int KB>
         __device__ inline int g_thread_index(int tid)
         {
             const int w = tid / 64;
-            if (w < 1 || w > 11 || (w % 4) == 0)
+            if constexpr (KB == 32)
             {
-                return -1;
+                if (w < 1 || w > 11 || (w % 4) == 0)
+                {
+                    return -1;
+                }
+                return (w - 1 - w / 4) * 64 + tid % 64;
             }
-            const int gi = (w - 1 - w / 4) * 64 + tid % 64;
-            return gi < kTriBlocks ? gi : -1;
+            else
+            {
+                // waves 1 .. up to the first Q wave (all waves when every thread holds Q)
+                constexpr int kQFirst = EigenShape<KB>::kThreads - 2 * KB * (KB / EigenShape<KB>::kQP);
+                constexpr int kGEnd = (kQFirst > 0) ? kQFirst : EigenShape<KB>::kThreads;
+                return (w < 1 || tid >= kGEnd) ? -1 : tid - 64;
+            }
         }
 
         // Column at a circle position in the next inner round.
+        template <int KW>
         __device__ inline int next_column(int c)
         {
-            return c == 0 ? 0 : (c == 1 ? kW - 1 : c - 1);
+            return c == 0 ? 0 : (c == 1 ? KW - 1 : c - 1);
         }
 
         // Circle position, in the previous round, of the column now at position p.
+        template <int KW>
         __device__ inline int previous_position(int p)
         {
-            return p == 0 ? 0 : (p == 1 ? kW - 1 : p - 1);
+            return p == 0 ? 0 : (p == 1 ? KW - 1 : p - 1);
         }
 
         // J_t^T [g11 g12; g21 g22] J_u for the rotations (ct, st) of the row pair and (cu, su) of the column pair
@@ -216,39 +297,41 @@ namespace jsvd
             n22 = fma(su, h21, cu * h22);
         }
 
-        // One inner round on a thread's Q registers: rotate its 8 position pairs, then move the positions one step
-        // along the circle (values crossing a segment boundary come from the neighbouring lanes).
-        template <typename T>
-        __device__ inline void q_round(T (&lo)[8], T (&hi)[8], const T (*cs)[2], int seg)
+        // One inner round on a thread's Q registers: rotate its QP position pairs, then move the positions one
+        // step along the circle (values crossing a segment boundary come from the neighbouring lanes).
+        template <typename T, int QP, int NSEG>
+        __device__ inline void q_round(T (&lo)[QP], T (&hi)[QP], const T (*cs)[2], int seg)
         {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < QP; ++i)
             {
-                const T c = cs[8 * seg + i][0], s = cs[8 * seg + i][1];
+                const T c = cs[QP * seg + i][0], s = cs[QP * seg + i][1];
                 const T x = lo[i], z = hi[i];
                 lo[i] = c * x - s * z;
                 hi[i] = s * x + c * z;
             }
-            const T out_lo = lo[7]; // position 8s+7 -> 8s+8 (next segment; segment 3: position 32, own hi[7])
-            const T out_hi = hi[0]; // position 63-8s -> 64-8s (previous segment; segment 0: position 1, own lo[1])
+            // position QP s + QP - 1 -> QP (s + 1) (next segment; last segment: position kb, its own hi[QP-1])
+            const T out_lo = lo[QP - 1];
+            // position kw - 1 - QP s -> kw - QP s (previous segment; segment 0: position 1, its own lo[1])
+            const T out_hi = hi[0];
             const T from_prev = __shfl_up(out_lo, 1, 64);
             const T from_next = __shfl_down(out_hi, 1, 64);
             const T lo0 = lo[0];
 #pragma unroll
-            for (int i = 7; i >= 1; --i)
+            for (int i = QP - 1; i >= 1; --i)
             {
                 lo[i] = lo[i - 1];
             }
 #pragma unroll
-            for (int i = 0; i < 7; ++i)
+            for (int i = 0; i < QP - 1; ++i)
             {
                 hi[i] = hi[i + 1];
             }
             lo[0] = (seg == 0) ? lo0 : from_prev; // position 0 never moves
-            hi[7] = (seg == 3) ? out_lo : from_next;
+            hi[QP - 1] = (seg == NSEG - 1) ? out_lo : from_next;
             if (seg == 0)
             {
-                lo[1] = out_hi; // position 63 -> 1
+                lo[1] = out_hi; // position kw - 1 -> 1
             }
         }
 
@@ -267,16 +350,28 @@ namespace jsvd
             }
         }
 
-        template <typename T>
-        __global__ __launch_bounds__(kEigenThreads) void block_eigen_kernel(
+        template <typename T, int KB>
+        __global__ __launch_bounds__(EigenShape<KB>::kThreads) void block_eigen_kernel(
             const T *__restrict__ partial, int slabs, int nb, int Nb, int round, double eps, T *__restrict__ q_out,
             int *__restrict__ apply_flag, int *__restrict__ rot_count, unsigned long long *__restrict__ offmax,
             const int *__restrict__ perm, double *__restrict__ norm2, const int *__restrict__ skip_if_zero)
         {
-            __shared__ T g[2][kW][kW + 1];              // G of the current round and of the next
+            constexpr int KW = 2 * KB;
+            constexpr int kPairs = KB;
+            constexpr int kThreads = EigenShape<KB>::kThreads;
+            constexpr int kQP = EigenShape<KB>::kQP;
+            constexpr int kSeg = kPairs / kQP;
+            constexpr int kQThreads = KW * kSeg;
+            constexpr int kQFirst = kThreads - kQThreads;
+            constexpr int kTri = kPairs * (kPairs + 1) / 2;
+            constexpr int kGThreads = EigenShape<KB>::kGThreads;
+            constexpr int kBPT = (kTri + kGThreads - 1) / kGThreads; // G blocks per G thread
+            static_assert(kQFirst % 64 == 0 && 64 % kSeg == 0 && kPairs <= 64, "eigen thread roles");
+            using L = GramLds<KB>;
+            __shared__ T g[2][L::kElems];               // G of the current round and of the next
             __shared__ T cs[2][kPairs][2];              // rotations of the current round and of the next
             __shared__ unsigned long long rmask[2];     // bit u: pair u rotates
-            __shared__ double red[kEigenThreads / 64];
+            __shared__ double red[kThreads / 64];
             __shared__ int any;
             __shared__ int needed;
             if (skip_if_zero != nullptr && *skip_if_zero == 0)
@@ -291,25 +386,27 @@ namespace jsvd
                 return;
             }
             const int tid = threadIdx.x;
-            // This thread's 2x2 block (t, u), t <= u, of the upper triangle of the 32 x 32 pair grid.
-            const int gi = g_thread_index(tid);
-            const bool gthr = gi >= 0;
-            int bt = 0, bu = 0;
-            if (gthr)
+            // This thread's 2x2 blocks (t, u), t <= u, of the upper triangle of the kb x kb pair grid.
+            const int gi0 = g_thread_index<KB>(tid);
+            const bool gthr = gi0 >= 0;
+            int btu[kBPT]; // t | u << 8, or -1 where the thread has no k-th block
+#pragma unroll
+            for (int k = 0; k < kBPT; ++k)
             {
-                int rem = gi;
-                while (rem >= kPairs - bt)
+                const int gi = gi0 + k * kGThreads;
+                int t = 0, rem = (gthr && gi < kTri) ? gi : 0;
+                while (rem >= kPairs - t)
                 {
-                    rem -= kPairs - bt;
-                    ++bt;
+                    rem -= kPairs - t;
+                    ++t;
                 }
-                bu = bt + rem;
+                btu[k] = (gthr && gi < kTri) ? (t | ((t + rem) << 8)) : -1;
             }
-            const T *base = partial + static_cast<int64_t>(slot) * slabs * (kW * kW);
-            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            const T *base = partial + static_cast<int64_t>(slot) * slabs * (KW * KW);
+            for (int e = tid; e < KW * KW; e += kThreads)
             {
-                const int i = e / kW;
-                const int j = e % kW;
+                const int i = e / KW;
+                const int j = e % KW;
                 if (i <= j)
                 {
                     // eight independent loads in flight per step; the summation order is fixed (slab order)
@@ -321,7 +418,7 @@ namespace jsvd
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
                         {
-                            part[k] = base[static_cast<int64_t>(s + k) * (kW * kW) + e];
+                            part[k] = base[static_cast<int64_t>(s + k) * (KW * KW) + e];
                         }
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
@@ -331,9 +428,9 @@ namespace jsvd
                     }
                     for (; s < slabs; ++s)
                     {
-                        acc += base[static_cast<int64_t>(s) * (kW * kW) + e];
+                        acc += base[static_cast<int64_t>(s) * (KW * KW) + e];
                     }
-                    g[0][i][j] = acc;
+                    g[0][L::at(i, j)] = acc;
                 }
             }
             if (tid == 0)
@@ -343,30 +440,30 @@ namespace jsvd
             }
             // Q = I in circle-position order (round 0 positions).
             const bool qthr = tid >= kQFirst;
-            const int qk = (tid - kQFirst) / 4;
-            const int seg = (tid - kQFirst) % 4;
-            T qlo[8], qhi[8];
+            const int qk = (tid - kQFirst) / kSeg;
+            const int seg = (tid - kQFirst) % kSeg;
+            T qlo[kQP], qhi[kQP];
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < kQP; ++i)
             {
-                qlo[i] = (dev::circle_at_fast(8 * seg + i, 0, kW) == qk) ? T(1) : T(0);
-                qhi[i] = (dev::circle_at_fast(kW - 1 - 8 * seg - i, 0, kW) == qk) ? T(1) : T(0);
+                qlo[i] = (dev::circle_at_fast(kQP * seg + i, 0, KW) == qk) ? T(1) : T(0);
+                qhi[i] = (dev::circle_at_fast(KW - 1 - kQP * seg - i, 0, KW) == qk) ? T(1) : T(0);
             }
             __syncthreads();
-            // Off-orthogonality of the pair's 64 columns before the sweep (convergence measure), and whether
-            // any pair passes the skip test at all: if none does, no rotation can happen in the sweep (G never
+            // Off-orthogonality of the pair's columns before the sweep (convergence measure), and whether any
+            // pair passes the skip test at all: if none does, no rotation can happen in the sweep (G never
             // changes) and the sweep is skipped outright.
             double best = 0.0;
             bool need = false;
-            for (int e = tid; e < kW * kW; e += kEigenThreads)
+            for (int e = tid; e < KW * KW; e += kThreads)
             {
-                const int i = e / kW;
-                const int j = e % kW;
+                const int i = e / KW;
+                const int j = e % KW;
                 if (i < j)
                 {
-                    const double gii = static_cast<double>(g[0][i][i]);
-                    const double gjj = static_cast<double>(g[0][j][j]);
-                    const double gij = static_cast<double>(g[0][i][j]);
+                    const double gii = static_cast<double>(g[0][L::at(i, i)]);
+                    const double gjj = static_cast<double>(g[0][L::at(j, j)]);
+                    const double gij = static_cast<double>(g[0][L::at(i, j)]);
                     best = fmax(best, dev::pair_cosine(gii, gjj, gij));
                     const double rhs = eps * sqrt(fmax(gii, 0.0) * fmax(gjj, 0.0));
                     need = need || (fabs(gij) > rhs && fabs(gij) > 1.0e-300);
@@ -383,18 +480,28 @@ namespace jsvd
             }
             __syncthreads();
             int my_rotations = 0;
-            int x1 = dev::circle_at_fast(bt, 0, kW), x2 = dev::circle_at_fast(kW - 1 - bt, 0, kW);
-            int z1 = dev::circle_at_fast(bu, 0, kW), z2 = dev::circle_at_fast(kW - 1 - bu, 0, kW);
+            // Columns at the block's circle positions: advanced incrementally with one block per thread; with
+            // several (kb = 64) they are recomputed per round, which keeps the registers for Q.
+            constexpr bool kTrack = kBPT == 1;
+            int x1[1], x2[1], z1[1], z2[1];
+            if constexpr (kTrack)
+            {
+                const int t = btu[0] & 255, u = (btu[0] >> 8) & 255;
+                x1[0] = dev::circle_at_fast(t, 0, KW);
+                x2[0] = dev::circle_at_fast(KW - 1 - t, 0, KW);
+                z1[0] = dev::circle_at_fast(u, 0, KW);
+                z2[0] = dev::circle_at_fast(KW - 1 - u, 0, KW);
+            }
             const bool run = needed != 0;
             // Wave 0: the rotation of pair pu of the current round and the three G entries it was formed from.
             const int pu = tid % kPairs;
             T my_c = 1, my_s = 0, gpp = 0, gqq = 0, gpq = 0;
             if (run && tid < 64)
             {
-                const int p1 = dev::circle_at_fast(pu, 0, kW), p2 = dev::circle_at_fast(kW - 1 - pu, 0, kW);
-                gpp = g[0][p1][p1];
-                gqq = g[0][p2][p2];
-                gpq = g[0][min(p1, p2)][max(p1, p2)];
+                const int p1 = dev::circle_at_fast(pu, 0, KW), p2 = dev::circle_at_fast(KW - 1 - pu, 0, KW);
+                gpp = g[0][L::at(p1, p1)];
+                gqq = g[0][L::at(p2, p2)];
+                gpq = g[0][L::at(p1, p2)];
                 double c, s;
                 const bool rot = block_rotation<T>(static_cast<double>(gpp), static_cast<double>(gqq),
                                                    static_cast<double>(gpq), eps, c, s);
@@ -413,63 +520,88 @@ namespace jsvd
                 }
             }
             __syncthreads();
-            for (int rr = 0; run && rr < kW - 1; ++rr)
+            for (int rr = 0; run && rr < KW - 1; ++rr)
             {
                 const int b = rr & 1;
-                T(*gc)[kW + 1] = g[b];
-                T *gn = &g[b ^ 1][0][0];
+                const T *gc = g[b];
+                T *gn = g[b ^ 1];
                 if (gthr)
                 {
-                    // J_t^T G J_u of this thread's block into the other buffer (copied when neither pair rotates)
-                    const int o11 = min(x1, z1) * (kW + 1) + max(x1, z1);
-                    const int o12 = min(x1, z2) * (kW + 1) + max(x1, z2);
-                    const int o21 = min(x2, z1) * (kW + 1) + max(x2, z1);
-                    const int o22 = min(x2, z2) * (kW + 1) + max(x2, z2);
-                    const T *gf = &gc[0][0];
-                    T n11 = gf[o11], n12 = gf[o12], n21 = gf[o21], n22 = gf[o22];
                     const unsigned long long m = rmask[b];
-                    if (((m >> bt) | (m >> bu)) & 1ull)
+#pragma unroll
+                    for (int k = 0; k < kBPT; ++k)
                     {
-                        rotate_block(cs[b][bt][0], cs[b][bt][1], cs[b][bu][0], cs[b][bu][1], n11, n12, n21, n22, n11,
-                                     n12, n21, n22);
+                        if (btu[k] >= 0)
+                        {
+                            const int t = btu[k] & 255, u = btu[k] >> 8;
+                            int c1, c2, c3, c4;
+                            if constexpr (kTrack)
+                            {
+                                c1 = x1[0];
+                                c2 = x2[0];
+                                c3 = z1[0];
+                                c4 = z2[0];
+                            }
+                            else
+                            {
+                                c1 = dev::circle_at_fast(t, rr, KW);
+                                c2 = dev::circle_at_fast(KW - 1 - t, rr, KW);
+                                c3 = dev::circle_at_fast(u, rr, KW);
+                                c4 = dev::circle_at_fast(KW - 1 - u, rr, KW);
+                            }
+                            // J_t^T G J_u of the block into the other buffer (copied when neither pair rotates)
+                            const int o11 = L::at(c1, c3);
+                            const int o12 = L::at(c1, c4);
+                            const int o21 = L::at(c2, c3);
+                            const int o22 = L::at(c2, c4);
+                            T n11 = gc[o11], n12 = gc[o12], n21 = gc[o21], n22 = gc[o22];
+                            if (((m >> t) | (m >> u)) & 1ull)
+                            {
+                                rotate_block(cs[b][t][0], cs[b][t][1], cs[b][u][0], cs[b][u][1], n11, n12, n21, n22,
+                                             n11, n12, n21, n22);
+                            }
+                            gn[o11] = n11;
+                            gn[o12] = n12;
+                            gn[o22] = n22;
+                            if (t != u)
+                            {
+                                gn[o21] = n21; // on the diagonal block entry 21 aliases entry 12
+                            }
+                            if constexpr (kTrack)
+                            {
+                                x1[0] = next_column<KW>(x1[0]);
+                                x2[0] = next_column<KW>(x2[0]);
+                                z1[0] = next_column<KW>(z1[0]);
+                                z2[0] = next_column<KW>(z2[0]);
+                            }
+                        }
                     }
-                    gn[o11] = n11;
-                    gn[o12] = n12;
-                    gn[o22] = n22;
-                    if (bt != bu)
-                    {
-                        gn[o21] = n21; // on the diagonal block entry 21 aliases entry 12
-                    }
-                    x1 = next_column(x1);
-                    x2 = next_column(x2);
-                    z1 = next_column(z1);
-                    z2 = next_column(z2);
                 }
                 if (qthr)
                 {
-                    q_round<T>(qlo, qhi, cs[b], seg);
+                    q_round<T, kQP, kSeg>(qlo, qhi, cs[b], seg);
                 }
-                if (tid < 64 && rr + 1 < kW - 1)
+                if (tid < 64 && rr + 1 < KW - 1)
                 {
                     // New diagonal entries of this lane's pair (its diagonal block; exact copy when not rotating).
                     T d11, d12, d21, d22;
                     rotate_block(my_c, my_s, my_c, my_s, gpp, gpq, gpq, gqq, d11, d12, d21, d22);
-                    // The pair of round rr+1: the columns now at positions pu and 63 - pu came from positions
-                    // pa and pb, i.e. from pairs ta (first column when pa < 32) and tb of round rr (ta != tb).
-                    const int pa = previous_position(pu), pb = previous_position(kW - 1 - pu);
-                    const int ta = pa < kPairs ? pa : kW - 1 - pa, tb = pb < kPairs ? pb : kW - 1 - pb;
+                    // The pair of round rr+1: the columns now at positions pu and kw - 1 - pu came from positions
+                    // pa and pb, i.e. from pairs ta (first column when pa < kb) and tb of round rr (ta != tb).
+                    const int pa = previous_position<KW>(pu), pb = previous_position<KW>(KW - 1 - pu);
+                    const int ta = pa < kPairs ? pa : KW - 1 - pa, tb = pb < kPairs ? pb : KW - 1 - pb;
                     const bool fa = pa < kPairs, fb = pb < kPairs;
                     const T a1 = __shfl(d11, ta, 64), a2 = __shfl(d22, ta, 64);
                     const T b1 = __shfl(d11, tb, 64), b2 = __shfl(d22, tb, 64);
                     const int t1 = min(ta, tb), t2 = max(ta, tb);
                     const T c1 = __shfl(my_c, t1, 64), s1 = __shfl(my_s, t1, 64);
                     const T c2 = __shfl(my_c, t2, 64), s2 = __shfl(my_s, t2, 64);
-                    // Their new off-diagonal entry: block (t1, t2) of round rr, as the G waves compute it.
-                    const int r1 = dev::circle_at_fast(t1, rr, kW), r2 = dev::circle_at_fast(kW - 1 - t1, rr, kW);
-                    const int k1 = dev::circle_at_fast(t2, rr, kW), k2 = dev::circle_at_fast(kW - 1 - t2, rr, kW);
+                    // Their new off-diagonal entry: block (t1, t2) of round rr, as the G threads compute it.
+                    const int r1 = dev::circle_at_fast(t1, rr, KW), r2 = dev::circle_at_fast(KW - 1 - t1, rr, KW);
+                    const int k1 = dev::circle_at_fast(t2, rr, KW), k2 = dev::circle_at_fast(KW - 1 - t2, rr, KW);
                     T f11, f12, f21, f22;
-                    rotate_block(c1, s1, c2, s2, gc[min(r1, k1)][max(r1, k1)], gc[min(r1, k2)][max(r1, k2)],
-                                 gc[min(r2, k1)][max(r2, k1)], gc[min(r2, k2)][max(r2, k2)], f11, f12, f21, f22);
+                    rotate_block(c1, s1, c2, s2, gc[L::at(r1, k1)], gc[L::at(r1, k2)], gc[L::at(r2, k1)],
+                                 gc[L::at(r2, k2)], f11, f12, f21, f22);
                     const bool row_first = (ta == t1) ? fa : fb; // role of the column from pair t1
                     const bool col_first = (ta == t1) ? fb : fa; // role of the column from pair t2
                     gpp = fa ? a1 : a2;
@@ -499,23 +631,23 @@ namespace jsvd
                 atomicAdd(&any, my_rotations);
             }
             __syncthreads();
-            // After kW - 1 moves the circle is back at its round-0 positions.
-            T *qo = q_out + static_cast<int64_t>(slot) * (kW * kW);
+            // After kw - 1 moves the circle is back at its round-0 positions.
+            T *qo = q_out + static_cast<int64_t>(slot) * (KW * KW);
             if (qthr)
             {
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
+                for (int i = 0; i < kQP; ++i)
                 {
-                    qo[qk * kW + dev::circle_at_fast(8 * seg + i, 0, kW)] = qlo[i]; // row-major Q[c][j]
-                    qo[qk * kW + dev::circle_at_fast(kW - 1 - 8 * seg - i, 0, kW)] = qhi[i];
+                    qo[qk * KW + dev::circle_at_fast(kQP * seg + i, 0, KW)] = qlo[i]; // row-major Q[c][j]
+                    qo[qk * KW + dev::circle_at_fast(KW - 1 - kQP * seg - i, 0, KW)] = qhi[i];
                 }
             }
             // squared norms of the pair's columns after the rotation (the diagonal of Q^T G Q), for the
             // norm ordering of the next sweep
-            if (norm2 != nullptr && tid < kW)
+            if (norm2 != nullptr && tid < KW)
             {
-                // the last inner round writes buffer (kW - 1) & 1
-                norm2[pair_column(perm, I, J, tid)] = static_cast<double>(g[run ? ((kW - 1) & 1) : 0][tid][tid]);
+                // the last inner round (kw - 2, even) writes buffer 1
+                norm2[pair_column<KB>(perm, I, J, tid)] = static_cast<double>(g[run ? 1 : 0][L::at(tid, tid)]);
             }
             if (tid == 0)
             {
@@ -527,7 +659,7 @@ namespace jsvd
                 if (offmax)
                 {
                     double m = 0.0;
-                    for (int w = 0; w < kEigenThreads / 64; ++w)
+                    for (int w = 0; w < kThreads / 64; ++w)
                     {
                         m = fmax(m, red[w]);
                     }
@@ -536,9 +668,22 @@ namespace jsvd
             }
         }
 
-        // Row-slab height of the apply kernel: 128 rows in FP32, 64 in FP64 (LDS for two workgroups per CU).
-        template <typename T>
-        constexpr int apply_rows() { return sizeof(T) == 4 ? 128 : 64; }
+        // Launch shape of the apply kernel per type and block width.  Waves split the slab's rows (kb <= 32:
+        // wave w computes rows [w H / 4, (w + 1) H / 4) x all kw columns) or, at kb = 64, its columns (wave w
+        // computes all H rows x columns [w kw / 4, (w + 1) kw / 4)), so each wave's B operands (its columns of Q)
+        // fit in registers.  H = slab rows: 128 in FP32 (64 at kb = 64), 64 in FP64 (32 at kb = 64), sized for
+        // two workgroups per CU (LDS and registers).
+        template <typename T, int KB>
+        struct ApplyShape
+        {
+            static constexpr int KW = 2 * KB;
+            static constexpr bool kColSplit = KB == 64;
+            static constexpr int H = (sizeof(T) == 4) ? (KB == 64 ? 64 : 128) : (KB == 64 ? 32 : 64);
+            static constexpr bool kQRegs = sizeof(T) == 4 || kColSplit;
+            static constexpr int NT = KW / 16;
+            static constexpr int RT = kColSplit ? H / 16 : H / 64; // 16-row tiles per wave
+            static constexpr int CT = kColSplit ? NT / 4 : NT;     // 16-column tiles per wave
+        };
 
         template <typename T>
         struct Vec16; // 16-byte vector of T
@@ -555,11 +700,11 @@ namespace jsvd
             using type = f64x2v;
         };
 
-        // Y = X Q for the row slabs of [X_I X_J] (the slabs of A, then those of V, apply_rows<T>() rows each).
-        // Workgroup y of a pair takes slabs y, y + gridDim.y, ...: Q is read once per workgroup (FP32: kept in
-        // registers as MFMA B operands; FP64: in LDS), the next slab's global loads are in flight while the
-        // current slab's MFMAs run, and the product is stored straight from the accumulators.
-        template <typename T>
+        // Y = X Q for the row slabs of [X_I X_J] (the slabs of A, then those of V, H rows each).  Workgroup y of a
+        // pair takes slabs y, y + gridDim.y, ...: Q is read once per workgroup (kept in registers as MFMA B
+        // operands, or in LDS for FP64 at kb <= 32), the next slab's global loads are in flight while the current
+        // slab's MFMAs run, and the product is stored straight from the accumulators.
+        template <typename T, int KB>
         __global__ __launch_bounds__(256) void block_apply_kernel(T *__restrict__ a, int64_t lda, int slabs_a,
                                                                   T *__restrict__ v, int64_t ldv, int slabs_v,
                                                                   int nb, int Nb, int round,
@@ -568,16 +713,19 @@ namespace jsvd
                                                                   const int *__restrict__ perm,
                                                                   const int *__restrict__ skip_if_zero)
         {
-            constexpr int H = apply_rows<T>();
+            using S = ApplyShape<T, KB>;
+            constexpr int KW = S::KW;
+            constexpr int H = S::H;
+            constexpr int RT = S::RT;
+            constexpr int CT = S::CT;
+            constexpr bool kQRegs = S::kQRegs;
             constexpr int kVec = 16 / static_cast<int>(sizeof(T)); // elements per 16-byte vector
             constexpr int kLd = H + kVec;                           // LDS row stride: 16-byte aligned rows
             constexpr int kColVecs = H / kVec;                      // vectors per column segment
-            constexpr int kPer = kW * kColVecs / 256;               // vectors per thread per slab
-            constexpr int kRT = H / 64;                             // 16-row tiles per wave
-            constexpr bool kQRegs = sizeof(T) == 4;
+            constexpr int kPer = KW * kColVecs / 256;               // vectors per thread per slab
             using V = typename Vec16<T>::type;
-            __shared__ __align__(16) T xs[kW][kLd];
-            __shared__ T qs[kQRegs ? 1 : kW][kW + 1];
+            __shared__ __align__(16) T xs[KW][kLd];
+            __shared__ T qs[kQRegs ? 1 : KW][KW + 1];
             if (skip_if_zero != nullptr && *skip_if_zero == 0)
             {
                 return;
@@ -598,25 +746,27 @@ namespace jsvd
             const int tid = threadIdx.x;
             const int lane = tid % 64;
             const int wave = tid / 64;
-            const T *qg = q_in + static_cast<int64_t>(slot) * (kW * kW); // row-major Q[c][j]
-            T bq[kQRegs ? kW / 4 : 1][4];
+            const int rb = S::kColSplit ? 0 : wave * 16 * RT; // first output row of this wave
+            const int cb = S::kColSplit ? wave * CT : 0;      // first output column tile of this wave
+            const T *qg = q_in + static_cast<int64_t>(slot) * (KW * KW); // row-major Q[c][j]
+            T bq[kQRegs ? KW / 4 : 1][kQRegs ? CT : 1];
             if constexpr (kQRegs)
             {
 #pragma unroll
-                for (int kk = 0; kk < kW / 4; ++kk)
+                for (int kk = 0; kk < KW / 4; ++kk)
                 {
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < CT; ++t)
                     {
-                        bq[kk][t] = qg[(4 * kk + lane / 16) * kW + 16 * t + lane % 16];
+                        bq[kk][t] = qg[(4 * kk + lane / 16) * KW + 16 * (cb + t) + lane % 16];
                     }
                 }
             }
             else
             {
-                for (int e = tid; e < kW * kW; e += 256)
+                for (int e = tid; e < KW * KW; e += 256)
                 {
-                    qs[e / kW][e % kW] = qg[e];
+                    qs[e / KW][e % KW] = qg[e];
                 }
             }
             // This thread's vectors of a slab: column (tid + 256 k) / kColVecs, rows ((tid + 256 k) % kColVecs) * kVec
@@ -626,7 +776,7 @@ namespace jsvd
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
             {
-                col_of[k] = static_cast<int>(pair_column(perm, I, J, (tid + 256 * k) / kColVecs));
+                col_of[k] = static_cast<int>(pair_column<KB>(perm, I, J, (tid + 256 * k) / kColVecs));
             }
             const int vrow = (tid % kColVecs) * kVec;
             V pre[kPer];
@@ -673,30 +823,28 @@ namespace jsvd
                         pre[k] = *reinterpret_cast<const V *>(xn + static_cast<int64_t>(col_of[k]) * ldn + r0n);
                     }
                 }
-                // wave: output rows [wave * 16 kRT, (wave + 1) * 16 kRT) x all 64 columns
-                typename Mfma<T>::Acc acc[kRT][4];
+                typename Mfma<T>::Acc acc[RT][CT];
 #pragma unroll
-                for (int a2 = 0; a2 < kRT; ++a2)
+                for (int a2 = 0; a2 < RT; ++a2)
                 {
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < CT; ++t)
                     {
                         acc[a2][t] = typename Mfma<T>::Acc{0, 0, 0, 0};
                     }
                 }
-                const int rb = wave * 16 * kRT;
 #pragma unroll
-                for (int kk = 0; kk < kW / 4; ++kk)
+                for (int kk = 0; kk < KW / 4; ++kk)
                 {
                     const int kc = 4 * kk + lane / 16;
-                    T av[kRT];
+                    T av[RT];
 #pragma unroll
-                    for (int a2 = 0; a2 < kRT; ++a2)
+                    for (int a2 = 0; a2 < RT; ++a2)
                     {
                         av[a2] = xs[kc][rb + 16 * a2 + lane % 16];
                     }
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < CT; ++t)
                     {
                         T bv;
                         if constexpr (kQRegs)
@@ -705,21 +853,21 @@ namespace jsvd
                         }
                         else
                         {
-                            bv = qs[kc][16 * t + lane % 16];
+                            bv = qs[kc][16 * (cb + t) + lane % 16];
                         }
 #pragma unroll
-                        for (int a2 = 0; a2 < kRT; ++a2)
+                        for (int a2 = 0; a2 < RT; ++a2)
                         {
                             acc[a2][t] = Mfma<T>::step(av[a2], bv, acc[a2][t]);
                         }
                     }
                 }
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int t = 0; t < CT; ++t)
                 {
-                    T *col = x + pair_column(perm, I, J, 16 * t + lane % 16) * ld + r0 + rb;
+                    T *col = x + pair_column<KB>(perm, I, J, 16 * (cb + t) + lane % 16) * ld + r0 + rb;
 #pragma unroll
-                    for (int a2 = 0; a2 < kRT; ++a2)
+                    for (int a2 = 0; a2 < RT; ++a2)
                     {
                         if constexpr (sizeof(T) == 4)
                         {
@@ -849,35 +997,75 @@ namespace jsvd
         }
     } // namespace
 
+    namespace
+    {
+        template <typename T, int KB>
+        void block_round_kb(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
+                            int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
+                            double *norm2, const int *skip_if_zero, hipStream_t stream)
+        {
+            const int pairs = g.Nb / 2;
+            // Slabs per Gram workgroup: fewer partials for the eigen kernel to reduce (its reduction runs on one
+            // CU per pair) while the Gram grid still covers the chip.
+            int per = 1;
+            while ((g.slabs_a + per - 1) / per > 8 && pairs * ((g.slabs_a + 2 * per - 1) / (2 * per)) >= 256)
+            {
+                per *= 2;
+            }
+            const int parts = (g.slabs_a + per - 1) / per;
+            hipLaunchKernelGGL((block_gram_kernel<T, KB>), dim3(pairs, parts), dim3(256), 0, stream,
+                               static_cast<const T *>(a), g.lda, perm, g.nb, g.Nb, round, g.slabs_a, per, parts,
+                               partial, skip_if_zero);
+            JSVD_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL((block_eigen_kernel<T, KB>), dim3(pairs), dim3(EigenShape<KB>::kThreads), 0, stream,
+                               static_cast<const T *>(partial), parts, g.nb, g.Nb, round, eps, q, apply_flag,
+                               rot_count, offmax, perm, norm2, skip_if_zero);
+            JSVD_HIP_CHECK(hipGetLastError());
+            // lda and ldv are multiples of kH, hence of the apply slab height; g.apply_wgs workgroups in total
+            const int h = ApplyShape<T, KB>::H;
+            const int sa = static_cast<int>(g.lda / h), sv = static_cast<int>(g.ldv / h);
+            const int per_pair = std::max(1, std::min(sa + sv, (g.apply_wgs + pairs - 1) / pairs));
+            hipLaunchKernelGGL((block_apply_kernel<T, KB>), dim3(pairs, per_pair), dim3(256), 0, stream, a, g.lda, sa,
+                               v, g.ldv, sv, g.nb, g.Nb, round, static_cast<const T *>(q),
+                               static_cast<const int *>(apply_flag), perm, skip_if_zero);
+            JSVD_HIP_CHECK(hipGetLastError());
+        }
+
+        template <typename T, int KB>
+        void block_eigen_kb(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
+        {
+            // A circle of 2 * pairs blocks at round 0 maps slot p to a valid block pair, so the kernel's pair p
+            // reads gram[p] (one partial) and writes q_out[p] and rotations[p]; no column map, norms or sweep
+            // counters.
+            const int Nb = 2 * pairs;
+            hipLaunchKernelGGL((block_eigen_kernel<T, KB>), dim3(pairs), dim3(EigenShape<KB>::kThreads), 0, stream,
+                               gram, 1, Nb, Nb, 0, eps, q_out, rotations, nullptr, nullptr, nullptr, nullptr,
+                               nullptr);
+            JSVD_HIP_CHECK(hipGetLastError());
+        }
+    } // namespace
+
     template <typename T>
     void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
                             int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
                             double *norm2, const int *skip_if_zero, hipStream_t stream)
     {
-        const int pairs = g.Nb / 2;
-        // Slabs per Gram workgroup: fewer partials for the eigen kernel to reduce (its reduction runs on one CU
-        // per pair) while the Gram grid still covers the chip.
-        int per = 1;
-        while ((g.slabs_a + per - 1) / per > 8 && pairs * ((g.slabs_a + 2 * per - 1) / (2 * per)) >= 256)
+        switch (g.kb)
         {
-            per *= 2;
+        case 16:
+            block_round_kb<T, 16>(g, a, v, round, eps, partial, q, apply_flag, rot_count, offmax, perm, norm2,
+                                  skip_if_zero, stream);
+            break;
+        case 32:
+            block_round_kb<T, 32>(g, a, v, round, eps, partial, q, apply_flag, rot_count, offmax, perm, norm2,
+                                  skip_if_zero, stream);
+            break;
+        case 64:
+            block_round_kb<T, 64>(g, a, v, round, eps, partial, q, apply_flag, rot_count, offmax, perm, norm2,
+                                  skip_if_zero, stream);
+            break;
+        default: throw_invalid("block width must be 16, 32 or 64.");
         }
-        const int parts = (g.slabs_a + per - 1) / per;
-        hipLaunchKernelGGL(block_gram_kernel<T>, dim3(pairs, parts), dim3(256), 0, stream, static_cast<const T *>(a),
-                           g.lda, perm, g.nb, g.Nb, round, g.slabs_a, per, parts, partial, skip_if_zero);
-        JSVD_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream,
-                           static_cast<const T *>(partial), parts, g.nb, g.Nb, round, eps, q, apply_flag, rot_count,
-                           offmax, perm, norm2, skip_if_zero);
-        JSVD_HIP_CHECK(hipGetLastError());
-        // lda and ldv are multiples of kH, hence of the apply slab height; g.apply_wgs workgroups in total
-        const int h = apply_rows<T>();
-        const int sa = static_cast<int>(g.lda / h), sv = static_cast<int>(g.ldv / h);
-        const int per_pair = std::max(1, std::min(sa + sv, (g.apply_wgs + pairs - 1) / pairs));
-        hipLaunchKernelGGL(block_apply_kernel<T>, dim3(pairs, per_pair), dim3(256), 0, stream, a, g.lda, sa, v, g.ldv,
-                           sv, g.nb, g.Nb, round, static_cast<const T *>(q), static_cast<const int *>(apply_flag),
-                           perm, skip_if_zero);
-        JSVD_HIP_CHECK(hipGetLastError());
     }
 
     template void launch_block_round<float>(const BlockGeometry &, float *, float *, int, double, float *, float *,
@@ -888,18 +1076,19 @@ namespace jsvd
                                              const int *, hipStream_t);
 
     template <typename T>
-    void launch_block_eigen(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
+    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
     {
-        // A circle of 2 * pairs blocks at round 0 maps slot p to a valid block pair, so the kernel's pair p reads
-        // gram[p] (one partial) and writes q_out[p] and rotations[p]; no column map, norms or sweep counters.
-        const int Nb = 2 * pairs;
-        hipLaunchKernelGGL(block_eigen_kernel<T>, dim3(pairs), dim3(kEigenThreads), 0, stream, gram, 1, Nb, Nb, 0, eps,
-                           q_out, rotations, nullptr, nullptr, nullptr, nullptr, nullptr);
-        JSVD_HIP_CHECK(hipGetLastError());
+        switch (kb)
+        {
+        case 16: block_eigen_kb<T, 16>(gram, pairs, eps, q_out, rotations, stream); break;
+        case 32: block_eigen_kb<T, 32>(gram, pairs, eps, q_out, rotations, stream); break;
+        case 64: block_eigen_kb<T, 64>(gram, pairs, eps, q_out, rotations, stream); break;
+        default: throw_invalid("block width must be 16, 32 or 64.");
+        }
     }
 
-    template void launch_block_eigen<float>(const float *, int, double, float *, int *, hipStream_t);
-    template void launch_block_eigen<double>(const double *, int, double, double *, int *, hipStream_t);
+    template void launch_block_eigen<float>(const float *, int, int, double, float *, int *, hipStream_t);
+    template void launch_block_eigen<double>(const double *, int, int, double, double *, int *, hipStream_t);
 
     void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
                              hipStream_t stream)

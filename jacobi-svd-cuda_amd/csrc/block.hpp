@@ -5,17 +5,20 @@
 
 namespace jsvd
 {
-    constexpr int kB = 32;      // block width (columns)
-    constexpr int kW = 2 * kB;  // columns of a block pair
-    constexpr int kH = 128;     // row-slab height of the Gram and apply tiles
+    constexpr int kB = 32;      // default block width (columns); 16 and 64 are the other instantiated widths
+    constexpr int kH = 128;     // row-slab unit: lda and ldv are multiples of it, Gram partials are per kH rows
+
+    // True for the instantiated block widths.
+    constexpr bool valid_block_width(int kb) { return kb == 16 || kb == 32 || kb == 64; }
 
     // Padded geometry of one matrix for the block kernels: lda / ldv are multiples of kH, the column count
-    // is padded to nb * kB with zero columns.
+    // is padded to nb * kb with zero columns.
     struct BlockGeometry
     {
         int64_t m = 0;      // rows of A
         int64_t n = 0;      // columns of A
-        int64_t n_pad = 0;  // nb * kB
+        int kb = kB;        // block width (16, 32 or 64); a block pair has 2 kb columns
+        int64_t n_pad = 0;  // nb * kb
         int nb = 0;         // blocks
         int Nb = 0;         // nb rounded up to even (round-robin circle size)
         int64_t lda = 0;    // rows of the padded A (multiple of kH)
@@ -24,13 +27,14 @@ namespace jsvd
         int slabs_v = 0;    // ldv / kH
         int apply_wgs = 512; // target workgroups of one apply launch (two per CU); the autotuner may change it
 
-        static BlockGeometry make(int64_t m, int64_t n)
+        static BlockGeometry make(int64_t m, int64_t n, int kb = kB)
         {
             BlockGeometry g;
             g.m = m;
             g.n = n;
-            g.nb = static_cast<int>((n + kB - 1) / kB);
-            g.n_pad = static_cast<int64_t>(g.nb) * kB;
+            g.kb = kb;
+            g.nb = static_cast<int>((n + kb - 1) / kb);
+            g.n_pad = static_cast<int64_t>(g.nb) * kb;
             g.Nb = g.nb + (g.nb & 1);
             g.lda = round_up(m, kH);
             g.ldv = round_up(g.n_pad, kH);
@@ -38,17 +42,32 @@ namespace jsvd
             g.slabs_v = static_cast<int>(g.ldv / kH);
             return g;
         }
+        [[nodiscard]] int kw() const { return 2 * kb; }
         [[nodiscard]] int rounds() const { return Nb - 1; }
         [[nodiscard]] int pairs() const { return Nb / 2; }
-        // Elements of the partial-Gram buffer: at most one partial per kH-row slab.
+        // Elements of the partial-Gram buffer: at most one kw x kw partial per kH-row slab and pair.
         [[nodiscard]] size_t partial_elems() const
         {
-            return static_cast<size_t>(pairs()) * static_cast<size_t>(slabs_a) * kW * kW;
+            return static_cast<size_t>(pairs()) * static_cast<size_t>(slabs_a) * kw() * kw();
         }
+        // Elements of the rotation buffer: one kw x kw matrix per pair.
+        [[nodiscard]] size_t q_elems() const { return static_cast<size_t>(pairs()) * kw() * kw(); }
     };
 
-    // One block round (Gram partials, reduction + inner sweep, apply to A and V).  partial holds
-    // g.partial_elems() elements, q pairs * kW * kW, apply_flag pairs ints.
+    // Block width for an n-column problem: `requested` (0 = the default kB), halved while it leaves fewer than
+    // two blocks (n <= width) and it is above 16.
+    inline int effective_block_width(int requested, int64_t n)
+    {
+        int kb = (requested == 0) ? kB : requested;
+        while (kb > 16 && n <= kb)
+        {
+            kb /= 2;
+        }
+        return kb;
+    }
+
+    // One block round (Gram partials, reduction + inner sweep, apply to A and V) at block width g.kb.  partial
+    // holds g.partial_elems() elements, q g.q_elems(), apply_flag pairs ints.
     // perm maps the sweep's logical column order to physical columns (A and V columns are addressed
     // through it); norm2 receives the squared norm of every column after its last rotation of the sweep.
     template <typename T>
@@ -56,11 +75,12 @@ namespace jsvd
                             int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
                             double *norm2, const int *skip_if_zero, hipStream_t stream);
 
-    // The block round's inner step alone, on `pairs` independent kW x kW Gram matrices (gram[p] row-major, upper
-    // triangle read): one cyclic Jacobi sweep in Gram space; q_out[p] (row-major) receives the accumulated
-    // rotation and rotations[p] the number of rotated column pairs.
+    // The block round's inner step alone, on `pairs` independent kw x kw Gram matrices (kw = 2 kb; gram[p]
+    // row-major, upper triangle read): one cyclic Jacobi sweep in Gram space; q_out[p] (row-major) receives the
+    // accumulated rotation and rotations[p] the number of rotated column pairs.
     template <typename T>
-    void launch_block_eigen(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream);
+    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations,
+                            hipStream_t stream);
 
     // Largest padded column count the one-workgroup ordering kernel sorts (LDS bound).
     constexpr int kSortMax = 8192;

@@ -36,20 +36,25 @@ namespace jsvd
         {
             return std::max(1e-6, 4.0 * std::sqrt(static_cast<double>(m)) * std::ldexp(1.0, -24));
         }
-        // The autotuner's apply grid size for this shape, when it measured one.
-        void apply_tuning(BlockGeometry &g)
+        // Geometry of an m x n solve: block width `requested` (16, 32 or 64), or for 0 the autotuner's choice
+        // when it measured this shape, else the default; the apply grid size measured with that width.
+        BlockGeometry solver_geometry(int64_t m, int64_t n, int requested)
         {
             Tuning t;
-            if (lookup_tuning(g.m, g.n, t))
+            const bool tuned = lookup_tuning(m, n, t);
+            const int kb = effective_block_width((requested == 0 && tuned) ? t.block_width : requested, n);
+            BlockGeometry g = BlockGeometry::make(m, n, kb);
+            if (tuned && t.block_width == kb)
             {
                 g.apply_wgs = t.apply_wgs;
             }
+            return g;
         }
     } // namespace
 
     BlockSolver::BlockSolver(int64_t m, int64_t n, const jsvd_config &cfg, const jsvd_mixed_config &mixed,
                              hipStream_t stream)
-        : g_(BlockGeometry::make(m, n)), cfg_(cfg), mixed_(mixed), stream_(stream)
+        : g_(solver_geometry(m, n, mixed.block_width)), cfg_(cfg), mixed_(mixed), stream_(stream)
     {
         if (g_.nb < 2)
         {
@@ -72,7 +77,7 @@ namespace jsvd
         JSVD_HIP_CHECK(hipMalloc(&a64_, sizeof(double) * a_elems));
         JSVD_HIP_CHECK(hipMalloc(&v64_, sizeof(double) * v_elems));
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
-        JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * g_.pairs() * kW * kW));
+        JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * g_.q_elems()));
         JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * g_.pairs()));
         JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * g_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * g_.n_pad));
@@ -211,17 +216,23 @@ namespace jsvd
 
     bool BlockSolver::reshape(int64_t m, int64_t n)
     {
-        BlockGeometry g = BlockGeometry::make(m, n);
-        const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
-        const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
-        const size_t part_elems = g.partial_elems();
-        if (g.nb < 2 || a_elems > cap_a_elems_ || v_elems > cap_v_elems_ || part_elems > cap_part_elems_ ||
-            g.pairs() > cap_pairs_ || g.n_pad > cap_npad_)
+        auto fits = [&](const BlockGeometry &x) {
+            return x.nb >= 2 && static_cast<size_t>(x.lda) * x.n_pad <= cap_a_elems_ &&
+                   static_cast<size_t>(x.ldv) * x.n_pad <= cap_v_elems_ && x.partial_elems() <= cap_part_elems_ &&
+                   x.pairs() <= cap_pairs_ && x.n_pad <= cap_npad_ && x.q_elems() <= cap_q_elems_;
+        };
+        BlockGeometry g = solver_geometry(m, n, mixed_.block_width);
+        if (!fits(g))
         {
-            return false;
+            // the preferred width needs more rotation or partial space than allocated: keep the current width
+            g = BlockGeometry::make(m, n, effective_block_width(g_.kb, n));
+            if (!fits(g))
+            {
+                return false;
+            }
         }
-        apply_tuning(g);
-        if (g.m != g_.m || g.n != g_.n)
+        const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
+        if (g.m != g_.m || g.n != g_.n || g.kb != g_.kb)
         {
             // padding rows and columns of the new extent must read as zero
             JSVD_HIP_CHECK(hipMemsetAsync(a64_, 0, sizeof(double) * a_elems, stream_));

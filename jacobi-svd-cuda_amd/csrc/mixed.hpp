@@ -59,6 +59,7 @@ namespace jsvd
         size_t cap_a_elems_ = 0;
         size_t cap_v_elems_ = 0;
         size_t cap_part_elems_ = 0;
+        size_t cap_q_elems_ = 0;
         int cap_pairs_ = 0;
         int64_t cap_npad_ = 0;
         double fp32_tol_requested_ = 0.0;

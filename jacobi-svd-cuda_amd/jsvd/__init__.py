@@ -64,7 +64,7 @@ class MixedConfig(ctypes.Structure):
         ("fp32_max_sweeps", ctypes.c_int32),
         ("fp32_tolerance", ctypes.c_double),
         ("orthogonalize_steps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("block_width", ctypes.c_int32),
     ]
 
 
@@ -84,10 +84,13 @@ class SolveInfo(ctypes.Structure):
 
 
 def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
-                 fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None) -> MixedConfig:
+                 fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None,
+                 block_width: Optional[int] = None) -> MixedConfig:
     mx = MixedConfig()
     lib().jsvd_mixed_config_default(ctypes.byref(mx))
     mx.path = path
+    if block_width is not None:
+        mx.block_width = block_width
     if fp32_max_sweeps is not None:
         mx.fp32_max_sweeps = fp32_max_sweeps
     if fp32_tolerance is not None:
@@ -183,7 +186,7 @@ def lib() -> ctypes.CDLL:
                                      _vp, _vp, ctypes.POINTER(SolveInfo)]
         L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
-        L.jsvd_block_eigen.argtypes = [ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
+        L.jsvd_block_eigen.argtypes = [ctypes.c_int32, ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp, _vp]
         for name in declared_symbols():
@@ -296,10 +299,11 @@ class Plan:
     def launch_params(self) -> dict:
         """The plan's current launch parameters (jsvd_plan_launch_params)."""
         L = lib()
-        L.jsvd_plan_launch_params.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int32)] * 4
-        vals = [ctypes.c_int32() for _ in range(4)]
+        L.jsvd_plan_launch_params.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int32)] * 5
+        vals = [ctypes.c_int32() for _ in range(5)]
         check(L.jsvd_plan_launch_params(self._h, *[ctypes.byref(v) for v in vals]))
-        return dict(zip(("round_threads", "round_ea", "round_ev", "apply_workgroups"), (v.value for v in vals)))
+        return dict(zip(("round_threads", "round_ea", "round_ev", "apply_workgroups", "block_width"),
+                        (v.value for v in vals)))
 
     def reshape(self, m: int, n: int, epsilon: float) -> None:
         """Re-targets the plan at m x n and epsilon inside its allocated capacity (jsvd_plan_reshape)."""
@@ -405,11 +409,12 @@ def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) 
 
 
 def block_eigen(gram, eps: float, q, rotations, stream=None) -> None:
-    """One Gram-space inner sweep on each 64 x 64 matrix of gram (pairs x 64 x 64, float32 or float64 CUDA
-    tensor): q receives the accumulated rotations, rotations (int32, pairs) the rotated-pair counts."""
+    """One Gram-space inner sweep on each kw x kw matrix of gram (pairs x kw x kw, kw = 32, 64 or 128, float32 or
+    float64 CUDA tensor): q receives the accumulated rotations, rotations (int32, pairs) the rotated-pair counts."""
     import torch
-    if gram.dim() != 3 or tuple(gram.shape[1:]) != (64, 64) or not gram.is_contiguous():
-        raise ValueError("gram must be a contiguous pairs x 64 x 64 tensor")
+    if gram.dim() != 3 or gram.shape[1] != gram.shape[2] or gram.shape[1] not in (32, 64, 128) \
+            or not gram.is_contiguous():
+        raise ValueError("gram must be a contiguous pairs x kw x kw tensor, kw in (32, 64, 128)")
     if q.shape != gram.shape or q.dtype != gram.dtype or not q.is_contiguous():
         raise ValueError("q must match gram")
     if rotations.dtype != torch.int32 or rotations.numel() != gram.shape[0]:
@@ -417,8 +422,8 @@ def block_eigen(gram, eps: float, q, rotations, stream=None) -> None:
     precision = PRECISION_F32 if gram.dtype == torch.float32 else PRECISION_F64
     if gram.dtype not in (torch.float32, torch.float64):
         raise ValueError("gram must be float32 or float64")
-    check(lib().jsvd_block_eigen(precision, _ptr(gram), gram.shape[0], eps, _ptr(q), _ptr(rotations),
-                                 _stream_handle(stream)))
+    check(lib().jsvd_block_eigen(precision, gram.shape[1] // 2, _ptr(gram), gram.shape[0], eps, _ptr(q),
+                                 _ptr(rotations), _stream_handle(stream)))
 
 
 # ------------------------------------------------------------------------------------------------
@@ -705,15 +710,18 @@ class TuneReport(ctypes.Structure):
         ("round_ea", ctypes.c_int32),
         ("round_ev", ctypes.c_int32),
         ("apply_workgroups", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("block_width", ctypes.c_int32),
         ("round_us_default", ctypes.c_double),
         ("round_us_best", ctypes.c_double),
+        ("block_sweep_us", ctypes.c_double * 3),
         ("block_round_us_default", ctypes.c_double),
         ("block_round_us_best", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        out = {f: getattr(self, f) for f, _ in self._fields_}
+        out["block_sweep_us"] = list(self.block_sweep_us)
+        return out
 
 
 def autotune(m: int, n: int, repetitions: int = 2, stream=None) -> TuneReport:

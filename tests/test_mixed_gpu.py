@@ -111,19 +111,21 @@ def test_block_paths_reject_small_n():
         jsvd.solve(gaussian(40, 32, 0), path=jsvd.PATH_MIXED)
 
 
+@pytest.mark.parametrize("kw", [32, 64, 128])
 @pytest.mark.parametrize("dtype,tol", [("float64", 1e-12), ("float32", 2e-4)])
-def test_block_eigen_matches_gram_space_sweep(torch_cuda, dtype, tol):
-    """The inner step of a block round (jsvd_block_eigen) against the numpy restatement of one Gram-space
-    sweep (tests/helpers.py): same rotation order and formula, so Q agrees to rounding (FP64) or to FP32
-    accumulation (FP32); Q is orthogonal and Q^T G Q equals the swept G."""
+def test_block_eigen_matches_gram_space_sweep(torch_cuda, dtype, tol, kw):
+    """The inner step of a block round (jsvd_block_eigen) at every block width (kw = 2 x 16, 32, 64 columns)
+    against the numpy restatement of one Gram-space sweep (tests/helpers.py): same rotation order and formula,
+    so Q agrees to rounding (FP64) or to FP32 accumulation (FP32); Q is orthogonal and Q^T G Q equals the
+    swept G."""
     import torch
     from helpers import gram_space_sweep
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(11 + kw)
     pairs = 5
     grams = []
     for p in range(pairs):
-        x = rng.standard_normal((300, 64)) * 10.0 ** (-(p + 1) * np.arange(64) / 63.0)
-        x[:, 63 - p] = 0.0  # a zero column never rotates
+        x = rng.standard_normal((300, kw)) * 10.0 ** (-(p + 1) * np.arange(kw) / (kw - 1.0))
+        x[:, kw - 1 - p] = 0.0  # a zero column never rotates
         grams.append(x.T @ x)
     g = np.stack(grams)
     tdt = getattr(torch, dtype)
@@ -153,6 +155,41 @@ def test_block_eigen_rejects_bad_arguments(torch_cuda):
         jsvd.block_eigen(g[:, :32, :], 1e-15, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))
     with pytest.raises(ValueError):
         jsvd.block_eigen(g, -1.0, torch.empty_like(g), torch.zeros(2, dtype=torch.int32, device="cuda"))
+    g48 = torch.zeros((2, 48, 48), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        jsvd.block_eigen(g48, 1e-15, torch.empty_like(g48), torch.zeros(2, dtype=torch.int32, device="cuda"))
+
+
+@pytest.mark.parametrize("block_width", [16, 32, 64])
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("m,n", [(512, 256), (300, 97)])
+def test_block_widths_against_oracle(torch_cuda, path, block_width, m, n):
+    """Every instantiated block width (16, 32, 64 columns; 300 x 97 leaves an odd block count at width 16 and
+    a padded last block at every width) solves to the same bar as the default width."""
+    a = gaussian(m, n, 3 + block_width)
+    mx = jsvd.mixed_config(path, block_width=block_width)
+    plan = jsvd.Plan(m, n, jsvd.JacobiSvdConfig(epsilon=1e-15, max_sweeps=40), path=path, mixed=mx)
+    got = plan.solve_host(a)
+    info = plan.last_info()
+    s_ref = np.linalg.svd(a, compute_uv=False)
+    s = sorted_desc(got.sigma)
+    assert info.converged == 1
+    assert np.max(np.abs(s - s_ref)) / s_ref[0] < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-13
+    assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
+
+
+def test_block_width_argument_checks(torch_cuda):
+    with pytest.raises(ValueError):
+        jsvd.Plan(100, 60, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, block_width=48))
+    # a width that would leave one block is halved: 40 columns at width 64 run at width 32
+    plan = jsvd.Plan(100, 40, jsvd.JacobiSvdConfig(epsilon=1e-15, max_sweeps=40), path=jsvd.PATH_BLOCK_F64,
+                     mixed=jsvd.mixed_config(jsvd.PATH_BLOCK_F64, block_width=64))
+    assert plan.launch_params()["block_width"] == 32
+    a = gaussian(100, 40, 9)
+    got = plan.solve_host(a)
+    s_ref = np.linalg.svd(a, compute_uv=False)
+    assert np.max(np.abs(sorted_desc(got.sigma) - s_ref)) / s_ref[0] < 1e-12
 
 
 @pytest.mark.parametrize("path", [jsvd.PATH_PLAIN_F64, jsvd.PATH_BLOCK_F64, jsvd.PATH_MIXED])
