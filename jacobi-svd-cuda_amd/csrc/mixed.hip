@@ -74,6 +74,7 @@ namespace jsvd
         cap_part_elems_ = part_elems;
         cap_pairs_ = g_.pairs();
         cap_npad_ = g_.n_pad;
+        cap_q_elems_ = g_.q_elems();
         JSVD_HIP_CHECK(hipMalloc(&a64_, sizeof(double) * a_elems));
         JSVD_HIP_CHECK(hipMalloc(&v64_, sizeof(double) * v_elems));
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
