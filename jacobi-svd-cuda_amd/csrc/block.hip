@@ -216,16 +216,18 @@ namespace jsvd
             static constexpr int kThreads = 1024, kQP = 8, kGThreads = 960;
         };
 
-        // LDS image of the symmetric kw x kw G, one slot per unordered entry.  kb <= 32: the upper triangle of a
-        // square array with a padded row.  kb = 64 (two buffers of a square array would not fit for FP64):
-        // entry {i, j} at row i, offset d = (j - i) mod kw for d < kw / 2 (row min(i, j) for d = kw / 2), else at
-        // row j, offset kw - d -- kw (kw / 2 + 1) slots.
-        template <int KB>
+        // LDS image of the symmetric kw x kw G, one slot per unordered entry.  The upper triangle of a square
+        // array with a padded row, except for FP64 at kb = 64, where two buffers of that would not fit: entry
+        // {i, j} at row i, offset d = (j - i) mod kw for d < kw / 2 (row min(i, j) for d = kw / 2), else at row
+        // j, offset kw - d, in rows of kw / 2 + 2 slots.  The even row length keeps both address steps that
+        // neighbouring lanes take (+1 along a row, row length - 1 across rows) odd, so a wave's accesses spread
+        // over all LDS banks (an odd row length makes the second step a multiple of the bank count).
+        template <typename T, int KB>
         struct GramLds
         {
             static constexpr int KW = 2 * KB;
-            static constexpr bool kCirc = KB == 64;
-            static constexpr int kStride = kCirc ? KW / 2 + 1 : KW + 1;
+            static constexpr bool kCirc = KB == 64 && sizeof(T) == 8;
+            static constexpr int kStride = kCirc ? KW / 2 + 2 : KW + 1;
             static constexpr int kElems = KW * kStride;
             __device__ static int at(int i, int j)
             {
@@ -367,7 +369,7 @@ namespace jsvd
             constexpr int kGThreads = EigenShape<KB>::kGThreads;
             constexpr int kBPT = (kTri + kGThreads - 1) / kGThreads; // G blocks per G thread
             static_assert(kQFirst % 64 == 0 && 64 % kSeg == 0 && kPairs <= 64, "eigen thread roles");
-            using L = GramLds<KB>;
+            using L = GramLds<T, KB>;
             __shared__ T g[2][L::kElems];               // G of the current round and of the next
             __shared__ T cs[2][kPairs][2];              // rotations of the current round and of the next
             __shared__ unsigned long long rmask[2];     // bit u: pair u rotates
@@ -480,17 +482,22 @@ namespace jsvd
             }
             __syncthreads();
             int my_rotations = 0;
-            // Columns at the block's circle positions: advanced incrementally with one block per thread; with
-            // several (kb = 64) they are recomputed per round, which keeps the registers for Q.
-            constexpr bool kTrack = kBPT == 1;
-            int x1[1], x2[1], z1[1], z2[1];
+            // Columns at the blocks' circle positions, advanced incrementally; FP64 at kb = 64 recomputes them per
+            // round instead, which keeps the registers for Q.
+            constexpr bool kTrack = !(KB == 64 && sizeof(T) == 8);
+            constexpr int kTB = kTrack ? kBPT : 1;
+            int x1[kTB], x2[kTB], z1[kTB], z2[kTB];
             if constexpr (kTrack)
             {
-                const int t = btu[0] & 255, u = (btu[0] >> 8) & 255;
-                x1[0] = dev::circle_at_fast(t, 0, KW);
-                x2[0] = dev::circle_at_fast(KW - 1 - t, 0, KW);
-                z1[0] = dev::circle_at_fast(u, 0, KW);
-                z2[0] = dev::circle_at_fast(KW - 1 - u, 0, KW);
+#pragma unroll
+                for (int k = 0; k < kBPT; ++k)
+                {
+                    const int t = btu[k] & 255, u = (btu[k] >> 8) & 255;
+                    x1[k] = dev::circle_at_fast(t, 0, KW);
+                    x2[k] = dev::circle_at_fast(KW - 1 - t, 0, KW);
+                    z1[k] = dev::circle_at_fast(u, 0, KW);
+                    z2[k] = dev::circle_at_fast(KW - 1 - u, 0, KW);
+                }
             }
             const bool run = needed != 0;
             // Wave 0: the rotation of pair pu of the current round and the three G entries it was formed from.
@@ -528,27 +535,12 @@ namespace jsvd
                 if (gthr)
                 {
                     const unsigned long long m = rmask[b];
-#pragma unroll
-                    for (int k = 0; k < kBPT; ++k)
+                    if constexpr (kBPT == 1)
                     {
-                        if (btu[k] >= 0)
+                        if (btu[0] >= 0)
                         {
-                            const int t = btu[k] & 255, u = btu[k] >> 8;
-                            int c1, c2, c3, c4;
-                            if constexpr (kTrack)
-                            {
-                                c1 = x1[0];
-                                c2 = x2[0];
-                                c3 = z1[0];
-                                c4 = z2[0];
-                            }
-                            else
-                            {
-                                c1 = dev::circle_at_fast(t, rr, KW);
-                                c2 = dev::circle_at_fast(KW - 1 - t, rr, KW);
-                                c3 = dev::circle_at_fast(u, rr, KW);
-                                c4 = dev::circle_at_fast(KW - 1 - u, rr, KW);
-                            }
+                            const int t = btu[0] & 255, u = btu[0] >> 8;
+                            const int c1 = x1[0], c2 = x2[0], c3 = z1[0], c4 = z2[0];
                             // J_t^T G J_u of the block into the other buffer (copied when neither pair rotates)
                             const int o11 = L::at(c1, c3);
                             const int o12 = L::at(c1, c4);
@@ -567,12 +559,59 @@ namespace jsvd
                             {
                                 gn[o21] = n21; // on the diagonal block entry 21 aliases entry 12
                             }
-                            if constexpr (kTrack)
+                            x1[0] = next_column<KW>(x1[0]);
+                            x2[0] = next_column<KW>(x2[0]);
+                            z1[0] = next_column<KW>(z1[0]);
+                            z2[0] = next_column<KW>(z2[0]);
+                        }
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int k = 0; k < kBPT; ++k)
+                        {
+                            if (btu[k] >= 0)
                             {
-                                x1[0] = next_column<KW>(x1[0]);
-                                x2[0] = next_column<KW>(x2[0]);
-                                z1[0] = next_column<KW>(z1[0]);
-                                z2[0] = next_column<KW>(z2[0]);
+                                const int t = btu[k] & 255, u = btu[k] >> 8;
+                                int c1, c2, c3, c4;
+                                if constexpr (kTrack)
+                                {
+                                    c1 = x1[k];
+                                    c2 = x2[k];
+                                    c3 = z1[k];
+                                    c4 = z2[k];
+                                }
+                                else
+                                {
+                                    c1 = dev::circle_at_fast(t, rr, KW);
+                                    c2 = dev::circle_at_fast(KW - 1 - t, rr, KW);
+                                    c3 = dev::circle_at_fast(u, rr, KW);
+                                    c4 = dev::circle_at_fast(KW - 1 - u, rr, KW);
+                                }
+                                const int o11 = L::at(c1, c3);
+                                const int o12 = L::at(c1, c4);
+                                const int o21 = L::at(c2, c3);
+                                const int o22 = L::at(c2, c4);
+                                T n11 = gc[o11], n12 = gc[o12], n21 = gc[o21], n22 = gc[o22];
+                                if (((m >> t) | (m >> u)) & 1ull)
+                                {
+                                    rotate_block(cs[b][t][0], cs[b][t][1], cs[b][u][0], cs[b][u][1], n11, n12, n21,
+                                                 n22, n11, n12, n21, n22);
+                                }
+                                gn[o11] = n11;
+                                gn[o12] = n12;
+                                gn[o22] = n22;
+                                if (t != u)
+                                {
+                                    gn[o21] = n21;
+                                }
+                                if constexpr (kTrack)
+                                {
+                                    x1[k] = next_column<KW>(x1[k]);
+                                    x2[k] = next_column<KW>(x2[k]);
+                                    z1[k] = next_column<KW>(z1[k]);
+                                    z2[k] = next_column<KW>(z2[k]);
+                                }
                             }
                         }
                     }
