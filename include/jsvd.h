@@ -99,7 +99,8 @@ int jsvd_plan_solve_host(jsvd_plan *plan, const double *a, double *u, double *si
 /*
  * Per-sweep diagnostics of the last solve on this plan: rotated-pair count and the largest
  * |gamma|/sqrt(alpha*beta) over all pairs of each sweep (the off-orthogonality measure of the
- * convergence test).  Arrays of length at least `capacity`; returns the number of sweeps written.
+ * convergence test).  Block and mixed plans list their FP32 sweeps first, then their FP64 sweeps.
+ * Arrays of length at least `capacity`; returns the number of sweeps written.
  */
 int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *offmax, int32_t capacity);
 
@@ -111,7 +112,7 @@ int jsvd_plan_sweep_stats(const jsvd_plan *plan, int32_t *rotations, double *off
  * ------------------------------------------------------------------------------------------------- */
 #define JSVD_PATH_PLAIN_F64 0 /* column-pair cyclic Jacobi, the reference algorithm */
 #define JSVD_PATH_BLOCK_F64 1 /* block Jacobi (16/32/64-column blocks), FP64 MFMA Gram and rotate */
-#define JSVD_PATH_MIXED 2     /* FP32 MFMA block sweeps, FP64 re-orthogonalisation, FP64 block finishing */
+#define JSVD_PATH_MIXED 2     /* FP64 QR, FP32 MFMA block sweeps, FP64 re-orthogonalisation, FP64 finishing */
 
 typedef struct jsvd_mixed_config
 {
@@ -121,6 +122,9 @@ typedef struct jsvd_mixed_config
     int32_t orthogonalize_steps; /* Newton-Schulz steps applied to the FP32 V in FP64 (default 2) */
     int32_t block_width;         /* columns per block: 16, 32 or 64; 0 (default) = jsvd_autotune's choice for the
                                     shape when it measured one, else 32; halved while n <= width */
+    int32_t qr_precondition;     /* mixed path: 1 (default) = factor A = Q R in FP64 first and sweep the n x n
+                                    R^T (sigma_j = |x_j|, V from the normalised columns X, U = Q V_M); 0 = sweep A */
+    int32_t reserved;
 } jsvd_mixed_config;
 
 /* Per-solve report of a plan. */

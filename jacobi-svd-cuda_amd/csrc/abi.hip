@@ -281,9 +281,9 @@ struct jsvd_plan
         if (block)
         {
             block->solve_device(a_rm, u_rm, sigma, v_rm, last_info);
-            last_sweeps = last_info.sweeps_fp64;
-            last_rot.clear();
-            last_off.clear();
+            last_rot = block->sweep_rotations();
+            last_off = block->sweep_offmax();
+            last_sweeps = static_cast<int>(last_rot.size());
             if (sweeps_out)
             {
                 *sweeps_out = last_info.sweeps_fp64;
@@ -562,6 +562,7 @@ void jsvd_mixed_config_default(jsvd_mixed_config *mixed)
     mixed->fp32_max_sweeps = 40;
     mixed->fp32_tolerance = 0.0;
     mixed->orthogonalize_steps = 2;
+    mixed->qr_precondition = 1;
 }
 
 namespace

@@ -2,19 +2,29 @@
 //
 // Mixed path, all on the device, no host round trip per sweep:
 //   A (row-major) -> A64 column-major, padded (LDS-tiled transpose)
-//   A32 = fp32(A64), V32 = I; FP32 block sweeps to the FP32 tolerance (MFMA, FP32 accumulation)
+//   QR: d_j = |a_j|, B = A D^-1; three Cholesky QR passes, the first with a diagonal shift:
+//       G = B^T B (+ s I), R_k = chol(G), B <- B R_k^-1; Q = B, R = R_3 R_2 R_1 D   (A = Q R)
+//   M = R^T (n x n); M32 = fp32(M), V32 = I; FP32 block sweeps to the FP32 tolerance (MFMA, FP32 accumulation)
 //   V64 = fp64(V32); Newton-Schulz V <- V (3I - V^T V) / 2 (FP64 GEMMs) until V is orthogonal in FP64
-//   A1 = A64 V64 (FP64 GEMM); FP64 block sweeps on (A1, V64) to epsilon
-//   sigma / U from A1 (column norms, fused transpose + divide), V64 -> row-major V
-// The GEMMs of the re-orthogonalisation and of A V are plain library GEMMs (rocBLAS dgemm).
+//   X = M V64 (FP64 GEMM); FP64 block sweeps on (X, V64) to epsilon
+//   sigma_j = |x_j|, V_A = X / sigma (row-major), U_A = Q V64 written row-major by one GEMM, columns with
+//   sigma_j <= epsilon zeroed as the reference zeroes them
+// The GEMMs, triangular solves and products and the Cholesky factorisations are plain library calls (rocBLAS
+// dgemm / dtrsm / dtrmm, rocSOLVER dpotrf).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
+// Yamamoto and Yanagisawa, "Shifted Cholesky QR for computing the QR factorization of ill-conditioned
+// matrices" (SIAM J. Sci. Comput. 42, 2020): shift s = 11 (m n + n (n + 1)) u |B|^2 with |B|_F^2 <= n for
+// unit columns; the column scaling keeps the factorisation's backward error columnwise, so column-graded
+// matrices keep their relative accuracy.
 #include "mixed.hpp"
 
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "autotune.hpp"
 #include "plain_f64.hpp"
@@ -32,12 +42,13 @@ namespace jsvd
             }
         }
 
-        double default_fp32_tolerance(int64_t m)
+        double default_fp32_tolerance(int64_t rows)
         {
-            return std::max(1e-6, 4.0 * std::sqrt(static_cast<double>(m)) * std::ldexp(1.0, -24));
+            return std::max(1e-6, 4.0 * std::sqrt(static_cast<double>(rows)) * std::ldexp(1.0, -24));
         }
-        // Geometry of an m x n solve: block width `requested` (16, 32 or 64), or for 0 the autotuner's choice
-        // when it measured this shape, else the default; the apply grid size measured with that width.
+
+        // Geometry of an m x n sweep problem: block width `requested` (16, 32 or 64), or for 0 the autotuner's
+        // choice when it measured this shape, else the default; the apply grid size measured with that width.
         BlockGeometry solver_geometry(int64_t m, int64_t n, int requested)
         {
             Tuning t;
@@ -50,38 +61,132 @@ namespace jsvd
             }
             return g;
         }
+
+        // Column j (one workgroup each, j < n_pad): dcol[j] = |a_j| (0 for padding columns), a_j <- a_j / |a_j|.
+        __global__ __launch_bounds__(256) void column_scale_kernel(double *__restrict__ a, int64_t lda, int64_t m,
+                                                                   int64_t n, double *__restrict__ dcol)
+        {
+            __shared__ double red[4];
+            const int64_t j = blockIdx.x;
+            double *col = a + j * lda;
+            double acc = 0.0;
+            if (j < n)
+            {
+                for (int64_t i = threadIdx.x; i < m; i += 256)
+                {
+                    acc += col[i] * col[i];
+                }
+            }
+            acc = dev::wave_sum(acc);
+            if ((threadIdx.x % 64) == 0)
+            {
+                red[threadIdx.x / 64] = acc;
+            }
+            __syncthreads();
+            const double d = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+            if (threadIdx.x == 0)
+            {
+                dcol[j] = d;
+            }
+            if (j < n && d > 0.0)
+            {
+                const double inv = 1.0 / d;
+                for (int64_t i = threadIdx.x; i < m; i += 256)
+                {
+                    col[i] *= inv;
+                }
+            }
+        }
+
+        __global__ void add_diagonal_kernel(double *__restrict__ g, int64_t ld, int64_t n, double s)
+        {
+            const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (i < n)
+            {
+                g[i * ld + i] += s;
+            }
+        }
+
+        // dst (n x n, ldd) <- upper triangle of src (lds), zeros below the diagonal.
+        __global__ void copy_upper_kernel(const double *__restrict__ src, int64_t lds, double *__restrict__ dst,
+                                          int64_t ldd, int64_t n)
+        {
+            const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (e >= n * n)
+            {
+                return;
+            }
+            const int64_t j = e / n;
+            const int64_t i = e % n;
+            dst[j * ldd + i] = (i <= j) ? src[j * lds + i] : 0.0;
+        }
+
+        // M (ldm x n_pad, column-major) <- (R D)^T for the upper triangular R (n x n, ldr) and column scales d:
+        // M[i][j] = R[j][i] d_i for j <= i < n, zero elsewhere (padding rows and columns included).
+        __global__ void transpose_scaled_kernel(const double *__restrict__ r, int64_t ldr,
+                                                const double *__restrict__ d, int64_t n, double *__restrict__ mo,
+                                                int64_t ldm, int64_t n_pad)
+        {
+            const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (e >= ldm * n_pad)
+            {
+                return;
+            }
+            const int64_t j = e / ldm;
+            const int64_t i = e % ldm;
+            mo[e] = (i < n && j < n && j <= i) ? r[i * ldr + j] * d[i] : 0.0;
+        }
+
+        // Row-major u (m x n): zero column j where sigma_j <= eps (the reference's U cutoff).
+        __global__ void mask_columns_kernel(double *__restrict__ u, int64_t m, int64_t n,
+                                            const double *__restrict__ sigma, double eps)
+        {
+            const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (e < m * n && !(sigma[e % n] > eps))
+            {
+                u[e] = 0.0;
+            }
+        }
+
+        unsigned grid_of(int64_t count) { return static_cast<unsigned>((count + 255) / 256); }
     } // namespace
+
+    void BlockSolver::set_geometry(int64_t m, int64_t n, const BlockGeometry &sweep)
+    {
+        g_ = sweep;
+        ga_ = BlockGeometry::make(m, n, sweep.kb);
+        ga_.apply_wgs = sweep.apply_wgs;
+    }
 
     BlockSolver::BlockSolver(int64_t m, int64_t n, const jsvd_config &cfg, const jsvd_mixed_config &mixed,
                              hipStream_t stream)
-        : g_(solver_geometry(m, n, mixed.block_width)), cfg_(cfg), mixed_(mixed), stream_(stream)
+        : cfg_(cfg), mixed_(mixed), stream_(stream)
     {
-        if (g_.nb < 2)
+        set_geometry(m, n, qr_first() ? solver_geometry(n, n, mixed_.block_width)
+                                      : solver_geometry(m, n, mixed_.block_width));
+        if (ga_.nb < 2)
         {
             throw_invalid("block and mixed paths need at least two 32-column blocks (n >= 33).");
         }
         fp32_tol_requested_ = mixed_.fp32_tolerance;
-        if (mixed_.fp32_tolerance <= 0.0)
-        {
-            mixed_.fp32_tolerance = default_fp32_tolerance(m);
-        }
         cap_ = std::max(cfg_.max_sweeps, mixed_.fp32_max_sweeps);
-        const size_t a_elems = static_cast<size_t>(g_.lda) * g_.n_pad;
-        const size_t v_elems = static_cast<size_t>(g_.ldv) * g_.n_pad;
-        const size_t part_elems = g_.partial_elems();
+        // buffers sized for the A-side geometry, which also covers the n x n sweep geometry and direct solves
+        const size_t a_elems = static_cast<size_t>(ga_.lda) * ga_.n_pad;
+        const size_t v_elems = static_cast<size_t>(ga_.ldv) * ga_.n_pad;
+        const size_t part_elems = ga_.partial_elems();
         cap_a_elems_ = a_elems;
         cap_v_elems_ = v_elems;
         cap_part_elems_ = part_elems;
-        cap_pairs_ = g_.pairs();
-        cap_npad_ = g_.n_pad;
-        cap_q_elems_ = g_.q_elems();
+        cap_pairs_ = ga_.pairs();
+        cap_npad_ = ga_.n_pad;
+        cap_q_elems_ = ga_.q_elems();
         JSVD_HIP_CHECK(hipMalloc(&a64_, sizeof(double) * a_elems));
         JSVD_HIP_CHECK(hipMalloc(&v64_, sizeof(double) * v_elems));
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
-        JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * g_.q_elems()));
-        JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * g_.pairs()));
-        JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * g_.n_pad));
-        JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * g_.n_pad));
+        JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * ga_.q_elems()));
+        JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * ga_.pairs()));
+        JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * ga_.n_pad));
+        JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * ga_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&rot_, sizeof(int) * 2 * cap_));
         JSVD_HIP_CHECK(hipMalloc(&off_, sizeof(unsigned long long) * 2 * cap_));
         JSVD_HIP_CHECK(hipHostMalloc(&h_rot_, sizeof(int) * 2 * cap_, hipHostMallocDefault));
@@ -96,10 +201,15 @@ namespace jsvd
         {
             JSVD_HIP_CHECK(hipMalloc(&a64b_, sizeof(double) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&w64_, sizeof(double) * v_elems));
-            JSVD_HIP_CHECK(hipMalloc(&gram64_, sizeof(double) * g_.n_pad * g_.n_pad));
+            JSVD_HIP_CHECK(hipMalloc(&gram64_, sizeof(double) * ga_.n_pad * ga_.n_pad));
+            JSVD_HIP_CHECK(hipMalloc(&racc_, sizeof(double) * ga_.n_pad * ga_.n_pad));
+            JSVD_HIP_CHECK(hipMalloc(&dcol_, sizeof(double) * ga_.n_pad));
+            JSVD_HIP_CHECK(hipMalloc(&chol_info_, sizeof(int)));
+            JSVD_HIP_CHECK(hipHostMalloc(&h_sigma_, sizeof(double) * ga_.n_pad, hipHostMallocDefault));
             JSVD_HIP_CHECK(hipMalloc(&a32_, sizeof(float) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&v32_, sizeof(float) * v_elems));
             JSVD_HIP_CHECK(hipMemsetAsync(a64b_, 0, sizeof(double) * a_elems, stream_));
+            JSVD_HIP_CHECK(hipMemsetAsync(w64_, 0, sizeof(double) * v_elems, stream_));
             rocblas_handle h = nullptr;
             blas_check(rocblas_create_handle(&h), "rocblas_create_handle");
             blas_ = h;
@@ -117,11 +227,13 @@ namespace jsvd
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
                         static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(off_),
-                        static_cast<void *>(perm_), static_cast<void *>(norm2_)})
+                        static_cast<void *>(perm_), static_cast<void *>(norm2_), static_cast<void *>(racc_),
+                        static_cast<void *>(dcol_), static_cast<void *>(chol_info_)})
         {
             (void)hipFree(p);
         }
         (void)hipHostFree(h_rot_);
+        (void)hipHostFree(h_sigma_);
         for (hipEvent_t e : done_)
         {
             if (e)
@@ -134,8 +246,8 @@ namespace jsvd
     // Block sweeps with the same device-side early exit and lagged host polling as the plain path.
     // Returns the executed sweep count (first rotation-free sweep included).
     template <typename T>
-    int BlockSolver::run_sweeps(T *a, T *v, double eps, int max_sweeps, int *rot, unsigned long long *off,
-                                double &last_off)
+    int BlockSolver::run_sweeps(const BlockGeometry &g, T *a, T *v, double eps, int max_sweeps, int *rot,
+                                unsigned long long *off, double &last_off)
     {
         JSVD_HIP_CHECK(hipMemsetAsync(rot, 0, sizeof(int) * max_sweeps, stream_));
         JSVD_HIP_CHECK(hipMemsetAsync(off, 0, sizeof(unsigned long long) * max_sweeps, stream_));
@@ -147,17 +259,17 @@ namespace jsvd
         // Sweep 0 runs in the incoming column order; every later sweep first orders the columns by descending
         // norm (the norms left by the previous sweep), which groups columns of similar size into the same
         // blocks and cuts the sweep count.  A and V columns are addressed through perm_, nothing moves.
-        launch_iota(perm_, g_.n_pad, stream_);
+        launch_iota(perm_, g.n_pad, stream_);
         for (int s = 0; s < max_sweeps; ++s)
         {
             const int *skip = (s > 0) ? rot + (s - 1) : nullptr;
             if (s > 0)
             {
-                launch_column_order(norm2_, g_.n_pad, perm_, skip, stream_);
+                launch_column_order(norm2_, g.n_pad, perm_, skip, stream_);
             }
-            for (int r = 0; r < g_.rounds(); ++r)
+            for (int r = 0; r < g.rounds(); ++r)
             {
-                launch_block_round<T>(g_, a, v, r, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip,
+                launch_block_round<T>(g, a, v, r, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip,
                                       stream_);
             }
             JSVD_HIP_CHECK(hipMemcpyAsync(h + s, rot + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -182,9 +294,16 @@ namespace jsvd
                 break;
             }
         }
-        unsigned long long bits = 0;
-        JSVD_HIP_CHECK(hipMemcpy(&bits, off + (executed - 1), sizeof(bits), hipMemcpyDeviceToHost));
-        std::memcpy(&last_off, &bits, sizeof(double));
+        std::vector<unsigned long long> bits(executed);
+        JSVD_HIP_CHECK(hipMemcpy(bits.data(), off, sizeof(unsigned long long) * executed, hipMemcpyDeviceToHost));
+        for (int s = 0; s < executed; ++s)
+        {
+            double x = 0.0;
+            std::memcpy(&x, &bits[s], sizeof(double));
+            stat_rot_.push_back(h[s]);
+            stat_off_.push_back(x);
+        }
+        last_off = stat_off_.back();
         return executed;
     }
 
@@ -222,69 +341,195 @@ namespace jsvd
                    static_cast<size_t>(x.ldv) * x.n_pad <= cap_v_elems_ && x.partial_elems() <= cap_part_elems_ &&
                    x.pairs() <= cap_pairs_ && x.n_pad <= cap_npad_ && x.q_elems() <= cap_q_elems_;
         };
-        BlockGeometry g = solver_geometry(m, n, mixed_.block_width);
-        if (!fits(g))
+        BlockGeometry sweep = qr_first() ? solver_geometry(n, n, mixed_.block_width)
+                                         : solver_geometry(m, n, mixed_.block_width);
+        if (!fits(BlockGeometry::make(m, n, sweep.kb)) || !fits(sweep))
         {
             // the preferred width needs more rotation or partial space than allocated: keep the current width
-            g = BlockGeometry::make(m, n, effective_block_width(g_.kb, n));
-            if (!fits(g))
+            const int kb = effective_block_width(g_.kb, n);
+            sweep = BlockGeometry::make(qr_first() ? n : m, n, kb);
+            if (!fits(BlockGeometry::make(m, n, kb)) || !fits(sweep))
             {
                 return false;
             }
         }
-        const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
-        if (g.m != g_.m || g.n != g_.n || g.kb != g_.kb)
+        const BlockGeometry a_side = BlockGeometry::make(m, n, sweep.kb);
+        if (a_side.m != ga_.m || a_side.n != ga_.n || a_side.kb != ga_.kb)
         {
             // padding rows and columns of the new extent must read as zero
+            const size_t a_elems = static_cast<size_t>(a_side.lda) * a_side.n_pad;
             JSVD_HIP_CHECK(hipMemsetAsync(a64_, 0, sizeof(double) * a_elems, stream_));
             if (a64b_)
             {
                 JSVD_HIP_CHECK(hipMemsetAsync(a64b_, 0, sizeof(double) * a_elems, stream_));
             }
         }
-        g_ = g;
-        mixed_.fp32_tolerance = (fp32_tol_requested_ > 0.0) ? fp32_tol_requested_ : default_fp32_tolerance(m);
+        set_geometry(m, n, sweep);
         return true;
+    }
+
+    void BlockSolver::fp32_phase(const BlockGeometry &g, const double *x64, double *y64, double tol,
+                                 jsvd_solve_info &info)
+    {
+        const int64_t x_elems = g.lda * g.n_pad;
+        const int64_t v_elems = g.ldv * g.n_pad;
+        launch_convert_f64_to_f32(x64, a32_, x_elems, stream_);
+        launch_identity_f32(v32_, g.ldv, g.n_pad, stream_);
+        const int fp32_cap = std::max(1, mixed_.fp32_max_sweeps);
+        info.sweeps_fp32 = run_sweeps<float>(g, a32_, v32_, tol, fp32_cap, rot_, off_, info.offmax_fp32);
+        launch_convert_f32_to_f64(v32_, v64_, v_elems, stream_);
+        orthogonalize_v(v64_, w64_);
+        rocblas_handle h = static_cast<rocblas_handle>(blas_);
+        const double one = 1.0, zero = 0.0;
+        blas_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, static_cast<rocblas_int>(g.lda),
+                                 static_cast<rocblas_int>(g.n_pad), static_cast<rocblas_int>(g.n_pad), &one, x64,
+                                 static_cast<rocblas_int>(g.lda), v64_, static_cast<rocblas_int>(g.ldv), &zero, y64,
+                                 static_cast<rocblas_int>(g.lda)),
+                   "dgemm X V");
+    }
+
+    bool BlockSolver::factor_qr()
+    {
+        rocblas_handle h = static_cast<rocblas_handle>(blas_);
+        const rocblas_int m = static_cast<rocblas_int>(ga_.m);
+        const rocblas_int n = static_cast<rocblas_int>(ga_.n);
+        const rocblas_int lda = static_cast<rocblas_int>(ga_.lda);
+        const rocblas_int ldr = static_cast<rocblas_int>(ga_.n_pad);
+        hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(ga_.n_pad)), dim3(256), 0, stream_, a64_,
+                           ga_.lda, ga_.m, ga_.n, dcol_);
+        JSVD_HIP_CHECK(hipGetLastError());
+        // |B|_2^2 <= |B|_F^2 <= n for unit columns
+        const double u = std::ldexp(1.0, -53);
+        const double shift = 11.0 * (static_cast<double>(m) * n + static_cast<double>(n) * (n + 1)) * u * n;
+        const double one = 1.0, zero = 0.0;
+        for (int pass = 0; pass < 3; ++pass)
+        {
+            blas_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, m, &one, a64_, lda,
+                                     a64_, lda, &zero, gram64_, ldr),
+                       "dgemm B^T B");
+            if (pass == 0)
+            {
+                hipLaunchKernelGGL(add_diagonal_kernel, dim3(grid_of(n)), dim3(256), 0, stream_, gram64_,
+                                   static_cast<int64_t>(ldr), static_cast<int64_t>(n), shift);
+                JSVD_HIP_CHECK(hipGetLastError());
+            }
+            blas_check(rocsolver_dpotrf(h, rocblas_fill_upper, n, gram64_, ldr, chol_info_), "rocsolver_dpotrf");
+            int status = 0;
+            JSVD_HIP_CHECK(hipMemcpyAsync(&status, chol_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+            JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
+            if (status != 0)
+            {
+                return false; // not positive definite: the caller solves A directly
+            }
+            blas_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                     rocblas_diagonal_non_unit, m, n, &one, gram64_, ldr, a64_, lda),
+                       "dtrsm B R^-1");
+            if (pass == 0)
+            {
+                hipLaunchKernelGGL(copy_upper_kernel, dim3(grid_of(static_cast<int64_t>(n) * n)), dim3(256), 0,
+                                   stream_, gram64_, static_cast<int64_t>(ldr), racc_, static_cast<int64_t>(ldr),
+                                   static_cast<int64_t>(n));
+                JSVD_HIP_CHECK(hipGetLastError());
+            }
+            else
+            {
+                // racc <- R_pass racc (upper times upper stays upper), through w64_ (free until the sweeps)
+                blas_check(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                         rocblas_diagonal_non_unit, n, n, &one, gram64_, ldr, racc_, ldr, w64_, ldr),
+                           "dtrmm R racc");
+                JSVD_HIP_CHECK(hipMemcpyAsync(racc_, w64_, sizeof(double) * ldr * n, hipMemcpyDeviceToDevice,
+                                              stream_));
+            }
+        }
+        // M = (R D)^T into a64b_ with the sweep geometry's padding
+        hipLaunchKernelGGL(transpose_scaled_kernel, dim3(grid_of(g_.lda * g_.n_pad)), dim3(256), 0, stream_, racc_,
+                           static_cast<int64_t>(ldr), dcol_, static_cast<int64_t>(n), a64b_, g_.lda, g_.n_pad);
+        JSVD_HIP_CHECK(hipGetLastError());
+        return true;
+    }
+
+    bool BlockSolver::solve_qr_first(const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                                     jsvd_solve_info &info)
+    {
+        launch_transpose(d_a, ga_.n, a64_, ga_.lda, ga_.m, ga_.n, stream_);
+        if (!factor_qr())
+        {
+            return false;
+        }
+        // sweeps on M = R^T (n x n): FP32 phase, then X = M V, FP64 phase
+        const double tol = (fp32_tol_requested_ > 0.0) ? fp32_tol_requested_ : default_fp32_tolerance(g_.m);
+        fp32_phase(g_, a64b_, w64_, tol, info);
+        info.sweeps_fp64 = run_sweeps<double>(g_, w64_, v64_, cfg_.epsilon, cfg_.max_sweeps, rot_ + cap_, off_ + cap_,
+                                              info.offmax_fp64);
+        info.converged = (h_rot_[cap_ + info.sweeps_fp64 - 1] == 0) ? 1 : 0;
+        // sigma_j = |x_j| and V_A = X / sigma (row-major), no cutoff: a column is normalised whenever sigma_j > 0
+        launch_u_sigma(w64_, g_.lda, g_.n, g_.n, 0.0, d_v, d_sigma, stream_);
+        // U_A = Q V_M, produced row-major directly: U^T = V_M^T Q^T is column-major n x m with leading dimension n
+        rocblas_handle h = static_cast<rocblas_handle>(blas_);
+        const double one = 1.0, zero = 0.0;
+        const rocblas_int m = static_cast<rocblas_int>(ga_.m), n = static_cast<rocblas_int>(ga_.n);
+        blas_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, n, m, n, &one, v64_,
+                                 static_cast<rocblas_int>(g_.ldv), a64_, static_cast<rocblas_int>(ga_.lda), &zero, d_u,
+                                 n),
+                   "dgemm (Q V)^T");
+        hipLaunchKernelGGL(mask_columns_kernel, dim3(grid_of(ga_.m * ga_.n)), dim3(256), 0, stream_, d_u, ga_.m,
+                           ga_.n, static_cast<const double *>(d_sigma), cfg_.epsilon);
+        JSVD_HIP_CHECK(hipGetLastError());
+        JSVD_HIP_CHECK(hipMemcpyAsync(h_sigma_, d_sigma, sizeof(double) * ga_.n, hipMemcpyDeviceToHost, stream_));
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (int64_t j = 0; j < ga_.n; ++j)
+        {
+            if (!(h_sigma_[j] > 0.0))
+            {
+                return false; // a zero column of X leaves V_A without a unit column there
+            }
+        }
+        return true;
+    }
+
+    void BlockSolver::solve_direct(const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                                   jsvd_solve_info &info)
+    {
+        // row-major m x n -> column-major lda x n (padding rows/columns keep their zeros)
+        launch_transpose(d_a, ga_.n, a64_, ga_.lda, ga_.m, ga_.n, stream_);
+        double *a_final = a64_;
+        if (mixed_.path == JSVD_PATH_MIXED)
+        {
+            const double tol = (fp32_tol_requested_ > 0.0) ? fp32_tol_requested_ : default_fp32_tolerance(ga_.m);
+            fp32_phase(ga_, a64_, a64b_, tol, info);
+            a_final = a64b_;
+        }
+        else
+        {
+            launch_pad_copy_f64(nullptr, 0, 0, 0, v64_, ga_.ldv, ga_.n_pad, true, stream_);
+        }
+        info.sweeps_fp64 = run_sweeps<double>(ga_, a_final, v64_, cfg_.epsilon, cfg_.max_sweeps, rot_ + cap_,
+                                              off_ + cap_, info.offmax_fp64);
+        info.converged = (h_rot_[cap_ + info.sweeps_fp64 - 1] == 0) ? 1 : 0;
+        launch_u_sigma(a_final, ga_.lda, ga_.m, ga_.n, cfg_.epsilon, d_u, d_sigma, stream_);
+        launch_transpose(v64_, ga_.ldv, d_v, ga_.n, ga_.n, ga_.n, stream_);
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
     void BlockSolver::solve_device(const double *d_a, double *d_u, double *d_sigma, double *d_v,
                                    jsvd_solve_info &info)
     {
         std::memset(&info, 0, sizeof(info));
+        stat_rot_.clear();
+        stat_off_.clear();
         info.path = mixed_.path;
-        // row-major m x n -> column-major lda x n (padding rows/columns keep their zeros)
-        launch_transpose(d_a, g_.n, a64_, g_.lda, g_.m, g_.n, stream_);
-        double *a_final = a64_;
-        if (mixed_.path == JSVD_PATH_MIXED)
+        if (qr_first())
         {
-            const int64_t a_elems = g_.lda * g_.n_pad;
-            const int64_t v_elems = g_.ldv * g_.n_pad;
-            launch_convert_f64_to_f32(a64_, a32_, a_elems, stream_);
-            launch_identity_f32(v32_, g_.ldv, g_.n_pad, stream_);
-            const int fp32_cap = std::max(1, mixed_.fp32_max_sweeps);
-            info.sweeps_fp32 = run_sweeps<float>(a32_, v32_, mixed_.fp32_tolerance, fp32_cap, rot_, off_,
-                                                 info.offmax_fp32);
-            launch_convert_f32_to_f64(v32_, v64_, v_elems, stream_);
-            orthogonalize_v(v64_, w64_);
-            rocblas_handle h = static_cast<rocblas_handle>(blas_);
-            const double one = 1.0, zero = 0.0;
-            blas_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none,
-                                     static_cast<rocblas_int>(g_.lda), static_cast<rocblas_int>(g_.n_pad),
-                                     static_cast<rocblas_int>(g_.n_pad), &one, a64_, static_cast<rocblas_int>(g_.lda),
-                                     v64_, static_cast<rocblas_int>(g_.ldv), &zero, a64b_,
-                                     static_cast<rocblas_int>(g_.lda)),
-                       "dgemm A V");
-            a_final = a64b_;
+            if (solve_qr_first(d_a, d_u, d_sigma, d_v, info))
+            {
+                return;
+            }
+            // the direct form on A (zero sigma or a Cholesky breakdown); the n x n problem's buffers are reused
+            std::memset(&info, 0, sizeof(info));
+            stat_rot_.clear();
+            stat_off_.clear();
+            info.path = mixed_.path;
         }
-        else
-        {
-            launch_pad_copy_f64(nullptr, 0, 0, 0, v64_, g_.ldv, g_.n_pad, true, stream_);
-        }
-        info.sweeps_fp64 = run_sweeps<double>(a_final, v64_, cfg_.epsilon, cfg_.max_sweeps, rot_ + cap_, off_ + cap_,
-                                              info.offmax_fp64);
-        info.converged = (h_rot_[cap_ + info.sweeps_fp64 - 1] == 0) ? 1 : 0;
-        launch_u_sigma(a_final, g_.lda, g_.m, g_.n, cfg_.epsilon, d_u, d_sigma, stream_);
-        launch_transpose(v64_, g_.ldv, d_v, g_.n, g_.n, g_.n, stream_);
-        JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
+        solve_direct(d_a, d_u, d_sigma, d_v, info);
     }
 } // namespace jsvd

@@ -8,10 +8,16 @@
 
 namespace jsvd
 {
-    // Device-resident block solver for one shape.  Path JSVD_PATH_BLOCK_F64 runs FP64 block sweeps only;
-    // JSVD_PATH_MIXED runs FP32 block sweeps (MFMA Gram and rotate, FP32 accumulation) to the FP32 tolerance,
-    // re-orthogonalises the FP32 V in FP64 (Newton-Schulz), forms A V in FP64 and finishes with FP64 block
-    // sweeps to the caller's epsilon.
+    // Device-resident block solver for one shape.  Path JSVD_PATH_BLOCK_F64 runs FP64 block sweeps only.
+    // Path JSVD_PATH_MIXED first factors A = Q R in FP64 (column-scaled shifted Cholesky QR, three passes) and
+    // solves the n x n problem M = R^T: FP32 block sweeps (MFMA Gram and rotate, FP32 accumulation) to the FP32
+    // tolerance, the FP32 V re-orthogonalised in FP64 (Newton-Schulz), M V formed in FP64, FP64 block sweeps to
+    // the caller's epsilon; then sigma_j = |x_j|, V_A = X / sigma (X = the converged M V) and U_A = Q V_M.  Sweeping
+    // R^T instead of A diagonalises R R^T, one QR-algorithm step further along than A^T A, which cuts the sweep
+    // count of both phases; the sweeps also run on n instead of m rows.  When a Cholesky factor breaks down or a
+    // sigma comes out exactly zero (V_A cannot be normalised) the solve is redone on A directly (FP32 sweeps,
+    // Newton-Schulz, A V, FP64 sweeps) with the reference's output conventions.  qr_precondition = 0 selects that
+    // direct form outright.
     class BlockSolver
     {
     public:
@@ -28,13 +34,28 @@ namespace jsvd
         bool reshape(int64_t m, int64_t n);
         void set_epsilon(double eps) { cfg_.epsilon = eps; }
         [[nodiscard]] const BlockGeometry &geometry() const { return g_; }
+        // Per-sweep rotated-pair counts and largest pair cosines of the last solve: the FP32 sweeps (mixed path),
+        // then the FP64 sweeps.
+        [[nodiscard]] const std::vector<int> &sweep_rotations() const { return stat_rot_; }
+        [[nodiscard]] const std::vector<double> &sweep_offmax() const { return stat_off_; }
 
     private:
         template <typename T>
-        int run_sweeps(T *a, T *v, double eps, int max_sweeps, int *rot, unsigned long long *off, double &last_off);
+        int run_sweeps(const BlockGeometry &g, T *a, T *v, double eps, int max_sweeps, int *rot,
+                       unsigned long long *off, double &last_off);
         void orthogonalize_v(double *v, double *w);
+        [[nodiscard]] bool qr_first() const { return mixed_.path == JSVD_PATH_MIXED && mixed_.qr_precondition != 0; }
+        void set_geometry(int64_t m, int64_t n, const BlockGeometry &sweep);
+        // FP32 sweeps on (x32 = fp32(x64), V32 = I) with geometry g, V64 <- orthogonalised fp64(V32), y = x64 V64.
+        void fp32_phase(const BlockGeometry &g, const double *x64, double *y64, double tol, jsvd_solve_info &info);
+        // Column-scaled shifted Cholesky QR of a64_ (m x n): a64_ <- Q, M = R^T into a64b_ (sweep geometry).
+        // False when a Cholesky factor breaks down.
+        bool factor_qr();
+        void solve_direct(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
+        bool solve_qr_first(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
 
-        BlockGeometry g_;
+        BlockGeometry ga_; // A-side geometry (m x n): input transpose, QR, direct solves
+        BlockGeometry g_;  // sweep geometry: n x n for the QR-first mixed path, else m x n
         jsvd_config cfg_;
         jsvd_mixed_config mixed_;
         hipStream_t stream_;
@@ -47,6 +68,10 @@ namespace jsvd
         float *a32_ = nullptr;
         float *v32_ = nullptr;
         double *partial_ = nullptr; // reused as float for the FP32 phase
+        double *racc_ = nullptr;    // [n_pad x n_pad] accumulated Cholesky QR factor
+        double *dcol_ = nullptr;    // [n_pad] column norms of A (QR column scaling)
+        int *chol_info_ = nullptr;  // [1] potrf status
+        double *h_sigma_ = nullptr; // [n_pad] pinned copy of sigma (zero check)
         double *q_ = nullptr;
         int *flags_ = nullptr;
         int *perm_ = nullptr;      // [n_pad] logical -> physical column of the current sweep
@@ -64,5 +89,7 @@ namespace jsvd
         int64_t cap_npad_ = 0;
         double fp32_tol_requested_ = 0.0;
         hipEvent_t done_[2] = {nullptr, nullptr};
+        std::vector<int> stat_rot_;
+        std::vector<double> stat_off_;
     };
 } // namespace jsvd

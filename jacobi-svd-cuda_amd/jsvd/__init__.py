@@ -65,6 +65,8 @@ class MixedConfig(ctypes.Structure):
         ("fp32_tolerance", ctypes.c_double),
         ("orthogonalize_steps", ctypes.c_int32),
         ("block_width", ctypes.c_int32),
+        ("qr_precondition", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -85,10 +87,12 @@ class SolveInfo(ctypes.Structure):
 
 def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
                  fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None,
-                 block_width: Optional[int] = None) -> MixedConfig:
+                 block_width: Optional[int] = None, qr_precondition: Optional[bool] = None) -> MixedConfig:
     mx = MixedConfig()
     lib().jsvd_mixed_config_default(ctypes.byref(mx))
     mx.path = path
+    if qr_precondition is not None:
+        mx.qr_precondition = int(bool(qr_precondition))
     if block_width is not None:
         mx.block_width = block_width
     if fp32_max_sweeps is not None:

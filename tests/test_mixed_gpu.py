@@ -221,3 +221,54 @@ def test_plan_reshape_equals_fresh_plans(torch_cuda, path):
     with pytest.raises(ValueError):
         big.reshape(300, 161, 1e-14)
     big.close()
+
+
+@pytest.mark.parametrize("m,n", [(1024, 256), (300, 97), (257, 257)])
+def test_mixed_qr_first_agrees_with_direct_form(torch_cuda, m, n):
+    """The mixed path's QR-first form (A = Q R, sweeps on R^T) and its direct form (sweeps on A) solve the same
+    problem to the same bar; on a geometric spectrum the QR-first form needs fewer FP32 and FP64 sweeps."""
+    from jsvd import workloads
+    a, s_true = workloads.tall_geometric(m, n) if m > n else (gaussian(m, n, 12), None)
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    res = {}
+    for qr in (True, False):
+        plan = jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=qr))
+        got = plan.solve_host(a)
+        res[qr] = (got, plan.last_info())
+        assert res[qr][1].converged == 1
+        assert rho_a(a, got.u, got.sigma, got.v) < 1e-13
+        assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
+    s1, s0 = sorted_desc(res[True][0].sigma), sorted_desc(res[False][0].sigma)
+    assert np.max(np.abs(s1 - s0)) / s0[0] < 1e-12
+    if s_true is not None:
+        assert np.max(np.abs(s1 - s_true)) / s_true[0] < 1e-12
+        assert res[True][1].sweeps_fp32 < res[False][1].sweeps_fp32
+        assert res[True][1].sweeps_fp64 < res[False][1].sweeps_fp64
+
+
+def test_mixed_rank_deficient_and_zero_inputs(torch_cuda):
+    """Exactly rank-deficient inputs: a zero matrix and a matrix with a zero column (the shifted Cholesky QR
+    breaks down and the solve falls back to the direct form), and a rank-60 matrix of 80 columns (sigma below
+    epsilon: those U columns are zero as in the reference; V stays orthogonal)."""
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    zero = np.zeros((100, 40))
+    got = jsvd.Plan(100, 40, cfg, path=jsvd.PATH_MIXED).solve_host(zero)
+    assert np.all(got.sigma == 0.0) and np.all(got.u == 0.0)
+    assert rho_orth(got.v) < 1e-12
+    a = gaussian(120, 50, 21)
+    a[:, 17] = 0.0
+    got = jsvd.Plan(120, 50, cfg, path=jsvd.PATH_MIXED).solve_host(a)
+    s_ref = np.linalg.svd(a, compute_uv=False)
+    assert np.max(np.abs(sorted_desc(got.sigma) - s_ref)) / s_ref[0] < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-13 and rho_orth(got.v) < 1e-11
+    rng = np.random.default_rng(22)
+    b = rng.standard_normal((200, 60)) @ rng.standard_normal((60, 80))
+    b /= np.linalg.norm(b, 2)  # sigma_max = 1: the 20 null singular values land at rounding level, below 1e-14
+    got = jsvd.Plan(200, 80, cfg, path=jsvd.PATH_MIXED).solve_host(b)
+    s = got.sigma
+    big = s > 1e-14
+    assert big.sum() == 60
+    assert np.all(got.u[:, ~big] == 0.0)
+    assert rho_orth(got.v) < 1e-11
+    assert rho_orth(got.u[:, big]) < 1e-11
+    assert rho_a(b, got.u, got.sigma, got.v) < 1e-13
