@@ -333,7 +333,9 @@ typedef struct jsvd_tune_report
     int32_t round_ea;              /* plain path: register chunks per thread for an A column (0 = streamed) */
     int32_t round_ev;              /* plain path: register chunks per thread for a V column (0 = streamed) */
     int32_t apply_workgroups;      /* block paths: total workgroups of one apply launch */
-    int32_t block_width;           /* block paths: chosen block width (16, 32 or 64; 0 if n <= 16) */
+    int32_t block_width;           /* block sweeps over m x n: chosen block width (16, 32 or 64; 0 if n <= 16) */
+    int32_t rt_block_width;        /* block sweeps over n x n (the mixed path's R^T): chosen block width */
+    int32_t rt_apply_workgroups;   /* block sweeps over n x n: chosen apply workgroups (0 if n <= 16) */
     double round_us_default;       /* us per round of the shape chosen by the built-in rule */
     double round_us_best;          /* us per round of the chosen shape */
     double block_sweep_us[3];      /* estimated us per FP32 block sweep at widths 16, 32, 64 (512 apply
@@ -347,7 +349,8 @@ typedef struct jsvd_tune_report
  * shape, every round-kernel launch shape that suits it (16 rounds of a first sweep each), the block width
  * (16 / 32 / 64 columns: up to 8 FP32 block rounds each, compared per sweep, i.e. per round times the rounds a
  * sweep takes at that width) and, at the chosen width, the block apply kernel's grid size (256 / 512 / 1024
- * workgroups), keeping the best of `repetitions` timings per candidate.  The winners are cached for the process: plans created or reshaped
+ * workgroups), keeping the best of `repetitions` timings per candidate.  The block choice is made for block
+ * sweeps over m x n and, when m > n, also over n x n (the mixed path sweeps the n x n R^T of A = Q R).  The winners are cached for the process: plans created or reshaped
  * to m x n afterwards use them, except that layout_transpose_mode FORCE_ENABLE / FORCE_DISABLE still fix the
  * round kernel's shape.  stream may be NULL (a private stream).  No counterpart in the reference, whose only
  * tuning is the layout-transpose threshold search (src/kernels.cu:1078-1127; auto_tune_layout_transpose_threshold

@@ -101,7 +101,7 @@ namespace jsvd
     RoundShape plan_round_shape(int64_t m, int64_t n, const jsvd_config &cfg)
     {
         Tuning t;
-        if (cfg.layout_transpose_mode == JSVD_LAYOUT_AUTO && lookup_tuning(m, n, t))
+        if (cfg.layout_transpose_mode == JSVD_LAYOUT_AUTO && lookup_tuning(m, n, t) && t.has_round)
         {
             return t.round;
         }
@@ -113,7 +113,7 @@ namespace jsvd
     RoundShape kernel_round_shape(int64_t m, int64_t n)
     {
         Tuning t;
-        return lookup_tuning(m, n, t) ? t.round : choose_round_shape(m, n, false);
+        return (lookup_tuning(m, n, t) && t.has_round) ? t.round : choose_round_shape(m, n, false);
     }
 
     int normalize_threads_per_block(int raw)

@@ -194,6 +194,82 @@ namespace jsvd
         return true;
     }
 
+    namespace
+    {
+        struct BlockChoice
+        {
+            int block_width = 0; // 0: the shape leaves fewer than two blocks at every width
+            int apply_wgs = 512;
+            double sweep_us[3] = {0.0, 0.0, 0.0};
+            double round_us_default = 0.0;
+            double round_us_best = 0.0;
+            int tried = 0;
+        };
+
+        // Block width by per-sweep time (round time x rounds per sweep, 512 apply workgroups), then the apply
+        // grid size at that width, for block sweeps over an m x n problem.
+        BlockChoice tune_block(int64_t m, int64_t n, int repetitions, hipStream_t stream)
+        {
+            BlockChoice c;
+            const int widths[3] = {16, 32, 64};
+            double best_sweep = -1.0;
+            for (int wi = 0; wi < 3; ++wi)
+            {
+                const BlockGeometry g = BlockGeometry::make(m, n, widths[wi]);
+                if (g.nb < 2)
+                {
+                    continue;
+                }
+                const double us = time_block_rounds(g, repetitions, stream);
+                ++c.tried;
+                c.sweep_us[wi] = us * g.rounds();
+                if (best_sweep < 0.0 || c.sweep_us[wi] < best_sweep)
+                {
+                    best_sweep = c.sweep_us[wi];
+                    c.block_width = widths[wi];
+                    c.round_us_default = us;
+                }
+            }
+            if (c.block_width == 0)
+            {
+                return c;
+            }
+            BlockGeometry g = BlockGeometry::make(m, n, c.block_width);
+            double best_us = c.round_us_default; // 512 workgroups, measured above
+            c.apply_wgs = g.apply_wgs;
+            for (int wgs : {256, 1024})
+            {
+                g.apply_wgs = wgs;
+                const double us = time_block_rounds(g, repetitions, stream);
+                ++c.tried;
+                if (us < best_us)
+                {
+                    best_us = us;
+                    c.apply_wgs = wgs;
+                }
+            }
+            c.round_us_best = best_us;
+            return c;
+        }
+
+        void store(int64_t m, int64_t n, const Tuning &t)
+        {
+            std::lock_guard<std::mutex> lock(g_mu);
+            Tuning &e = g_table[{m, n}];
+            if (t.has_round)
+            {
+                e.round = t.round;
+                e.has_round = true;
+            }
+            if (t.has_block)
+            {
+                e.block_width = t.block_width;
+                e.apply_wgs = t.apply_wgs;
+                e.has_block = true;
+            }
+        }
+    } // namespace
+
     Tuning run_autotune(int64_t m, int64_t n, int repetitions, hipStream_t stream, jsvd_tune_report *report)
     {
         validate_shape(m, n);
@@ -222,7 +298,7 @@ namespace jsvd
                 // same data for every candidate: A from the seeded fill (padding rows zero), V = I
                 // (the kernel reads the padding row of an odd m as zero whatever it holds)
                 double us = -1.0;
-                for (int rep = 0; rep < repetitions; ++rep)
+                for (int r = 0; r < repetitions; ++r)
                 {
                     JSVD_HIP_CHECK(
                         hipMemcpyAsync(a.p, a0.p, sizeof(double) * lda * n, hipMemcpyDeviceToDevice, stream));
@@ -245,54 +321,46 @@ namespace jsvd
                 }
             }
             rep.round_us_best = best_us;
+            best.has_round = true;
         }
-        // block paths: the block width (per-sweep time), then the apply grid size at that width (FP32 rounds)
-        const int widths[3] = {16, 32, 64};
-        double best_sweep = -1.0;
-        for (int wi = 0; wi < 3; ++wi)
+        // block paths over m x n (the block FP64 path and the mixed path's direct form)
+        const BlockChoice bc = tune_block(m, n, repetitions, stream);
+        rep.candidates_tried += bc.tried;
+        if (bc.block_width != 0)
         {
-            const BlockGeometry g = BlockGeometry::make(m, n, widths[wi]);
-            if (g.nb < 2)
+            best.block_width = bc.block_width;
+            best.apply_wgs = bc.apply_wgs;
+            best.has_block = true;
+            rep.block_width = bc.block_width;
+            for (int k = 0; k < 3; ++k)
             {
-                continue;
+                rep.block_sweep_us[k] = bc.sweep_us[k];
             }
-            const double us = time_block_rounds(g, repetitions, stream);
-            ++rep.candidates_tried;
-            rep.block_sweep_us[wi] = us * g.rounds();
-            if (best_sweep < 0.0 || rep.block_sweep_us[wi] < best_sweep)
-            {
-                best_sweep = rep.block_sweep_us[wi];
-                best.block_width = widths[wi];
-                rep.block_round_us_default = us;
-            }
+            rep.block_round_us_default = bc.round_us_default;
+            rep.block_round_us_best = bc.round_us_best;
         }
-        if (best_sweep >= 0.0)
+        // block sweeps over the n x n R^T of the mixed path's QR-first form (the same problem when m == n)
+        BlockChoice sq = bc;
+        if (m != n)
         {
-            BlockGeometry g = BlockGeometry::make(m, n, best.block_width);
-            double best_us = rep.block_round_us_default; // 512 workgroups, measured above
-            best.apply_wgs = g.apply_wgs;
-            for (int wgs : {256, 1024})
-            {
-                g.apply_wgs = wgs;
-                const double us = time_block_rounds(g, repetitions, stream);
-                ++rep.candidates_tried;
-                if (us < best_us)
-                {
-                    best_us = us;
-                    best.apply_wgs = wgs;
-                }
-            }
-            rep.block_round_us_best = best_us;
-            rep.block_width = best.block_width;
+            sq = tune_block(n, n, repetitions, stream);
+            rep.candidates_tried += sq.tried;
         }
+        rep.rt_block_width = sq.block_width;
+        rep.rt_apply_workgroups = (sq.block_width != 0) ? sq.apply_wgs : 0;
         JSVD_HIP_CHECK(hipStreamSynchronize(stream));
         rep.round_threads = best.round.threads;
         rep.round_ea = best.round.ea;
         rep.round_ev = best.round.ev;
         rep.apply_workgroups = best.apply_wgs;
+        store(m, n, best);
+        if (m != n && sq.block_width != 0)
         {
-            std::lock_guard<std::mutex> lock(g_mu);
-            g_table[{m, n}] = best;
+            Tuning t;
+            t.block_width = sq.block_width;
+            t.apply_wgs = sq.apply_wgs;
+            t.has_block = true;
+            store(n, n, t);
         }
         if (report)
         {

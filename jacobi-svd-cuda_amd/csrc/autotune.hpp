@@ -8,12 +8,15 @@ namespace jsvd
 {
     struct Tuning
     {
-        RoundShape round{};  // plain path: round-kernel launch shape
-        int apply_wgs = 512; // block paths: total workgroups of one apply launch
-        int block_width = 32; // block paths: columns per block
+        RoundShape round{};   // plain path: round-kernel launch shape (valid when has_round)
+        int apply_wgs = 512;  // block sweeps over m x n: total workgroups of one apply launch (has_block)
+        int block_width = 32; // block sweeps over m x n: columns per block (has_block)
+        bool has_round = false;
+        bool has_block = false;
     };
 
-    // The cached choice for m x n, if jsvd_autotune measured it in this process.
+    // The cached choices for m x n, if jsvd_autotune measured any in this process (jsvd_autotune(m, n) also
+    // records the block choice for n x n, the sweep shape of the mixed path's QR-first form).
     bool lookup_tuning(int64_t m, int64_t n, Tuning &out);
 
     // Measures the candidates for m x n on `stream`, caches and returns the fastest.

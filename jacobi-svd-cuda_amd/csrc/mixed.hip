@@ -52,7 +52,7 @@ namespace jsvd
         BlockGeometry solver_geometry(int64_t m, int64_t n, int requested)
         {
             Tuning t;
-            const bool tuned = lookup_tuning(m, n, t);
+            const bool tuned = lookup_tuning(m, n, t) && t.has_block;
             const int kb = effective_block_width((requested == 0 && tuned) ? t.block_width : requested, n);
             BlockGeometry g = BlockGeometry::make(m, n, kb);
             if (tuned && t.block_width == kb)

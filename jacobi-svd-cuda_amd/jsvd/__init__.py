@@ -715,6 +715,8 @@ class TuneReport(ctypes.Structure):
         ("round_ev", ctypes.c_int32),
         ("apply_workgroups", ctypes.c_int32),
         ("block_width", ctypes.c_int32),
+        ("rt_block_width", ctypes.c_int32),
+        ("rt_apply_workgroups", ctypes.c_int32),
         ("round_us_default", ctypes.c_double),
         ("round_us_best", ctypes.c_double),
         ("block_sweep_us", ctypes.c_double * 3),

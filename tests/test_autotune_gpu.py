@@ -41,9 +41,13 @@ def test_report_and_plain_plan_use_choice(torch_cuda):
     after = jsvd.Plan(m, n, _plain_cfg()).launch_params()
     assert (after["round_threads"], after["round_ea"], after["round_ev"]) == (
         rep.round_threads, rep.round_ea, rep.round_ev)
+    # the mixed path sweeps the n x n R^T, the block FP64 path sweeps A itself
     mixed = jsvd.Plan(m, n, _plain_cfg(), path=jsvd.PATH_MIXED).launch_params()
-    assert mixed["apply_workgroups"] == rep.apply_workgroups and mixed["round_threads"] == 0
-    assert mixed["block_width"] == rep.block_width
+    assert mixed["round_threads"] == 0
+    assert (mixed["apply_workgroups"], mixed["block_width"]) == (rep.rt_apply_workgroups, rep.rt_block_width)
+    blk = jsvd.Plan(m, n, _plain_cfg(), path=jsvd.PATH_BLOCK_F64).launch_params()
+    assert (blk["apply_workgroups"], blk["block_width"]) == (rep.apply_workgroups, rep.block_width)
+    assert rep.rt_block_width in (16, 32, 64) and rep.rt_apply_workgroups in (256, 512, 1024)
     # an explicit width wins over the tuned one
     other = 16 if rep.block_width != 16 else 32
     forced_w = jsvd.Plan(m, n, _plain_cfg(), path=jsvd.PATH_MIXED,
@@ -77,7 +81,7 @@ def test_tuned_mixed_plan_and_reshape(torch_cuda):
     assert (p0["apply_workgroups"], p0["block_width"]) == (512, 32)  # no tuning for 2048 x 512: built-in
     plan.reshape(m, n, 1e-15)
     p1 = plan.launch_params()
-    assert (p1["apply_workgroups"], p1["block_width"]) == (rep.apply_workgroups, rep.block_width)
+    assert (p1["apply_workgroups"], p1["block_width"]) == (rep.rt_apply_workgroups, rep.rt_block_width)
     a = gaussian(m, n, 5)
     got = plan.solve_host(a)
     s_ref = np.linalg.svd(a, compute_uv=False)
