@@ -174,6 +174,16 @@ int jsvd_plan_launch_params(const jsvd_plan *plan, int32_t *round_threads, int32
 int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg, const jsvd_mixed_config *mixed,
                    double *u, double *sigma, double *v, jsvd_solve_info *info);
 
+/*
+ * Cholesky factorisation A = R^T R of the symmetric positive definite n x n FP64 matrix a (device, column-major,
+ * leading dimension lda >= n rounded up to 64): the upper triangle of a is overwritten by R, the padding rows
+ * and columns up to the next multiple of 64 are set to the identity (so a needs that room), the strictly lower
+ * triangle is left as it was.  *info (device int32) becomes 0, or 1 + the first row whose pivot was not
+ * positive.  Tiled right-looking factorisation (64 x 64 tiles, FP64 MFMA trailing updates); the building block
+ * of the mixed path's Cholesky QR.  No reference counterpart.
+ */
+int jsvd_cholesky_f64(double *a, int64_t lda, int64_t n, int32_t *info, void *stream);
+
 /* ---------------------------------------------------------------------------------------------------
  * Batched path: many independent matrices of different shapes solved by one persistent launch.  No
  * reference counterpart as a batch (the reference solves one matrix per call, pipeline.cpp:711-739); per

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "autotune.hpp"
+#include "cholesky.hpp"
 #include "common.hpp"
 #include "mixed.hpp"
 #include "plain_f64.hpp"
@@ -637,6 +638,16 @@ int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon)
         require(epsilon > 0.0, "epsilon must be positive.");
         jsvd::validate_shape(m, n);
         require(plan->reshape(m, n, epsilon), "shape does not fit the plan's capacity.");
+    });
+}
+
+int jsvd_cholesky_f64(double *a, int64_t lda, int64_t n, int32_t *info, void *stream)
+{
+    return jsvd::guarded([&] {
+        require(n >= 1 && n <= (int64_t{1} << 20), "n out of range.");
+        require(lda >= jsvd::cholesky_padded_size(n), "lda must be at least n rounded up to 64.");
+        require(a && info, "a and info must not be null.");
+        jsvd::launch_cholesky_upper(a, lda, n, info, static_cast<hipStream_t>(stream));
     });
 }
 

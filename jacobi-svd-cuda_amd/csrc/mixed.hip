@@ -9,8 +9,8 @@
 //   X = M V64 (FP64 GEMM); FP64 block sweeps on (X, V64) to epsilon
 //   sigma_j = |x_j|, V_A = X / sigma (row-major), U_A = Q V64 written row-major by one GEMM, columns with
 //   sigma_j <= epsilon zeroed as the reference zeroes them
-// The GEMMs, triangular solves and products and the Cholesky factorisations are plain library calls (rocBLAS
-// dgemm / dtrsm / dtrmm, rocSOLVER dpotrf).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
+// The GEMMs, triangular solves and products are plain library calls (rocBLAS dgemm / dtrsm / dtrmm); the
+// Cholesky factorisations are this library's tiled kernels (cholesky.hip).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
 // Yamamoto and Yanagisawa, "Shifted Cholesky QR for computing the QR factorization of ill-conditioned
 // matrices" (SIAM J. Sci. Comput. 42, 2020): shift s = 11 (m n + n (n + 1)) u |B|^2 with |B|_F^2 <= n for
 // unit columns; the column scaling keeps the factorisation's backward error columnwise, so column-graded
@@ -18,7 +18,6 @@
 #include "mixed.hpp"
 
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -27,6 +26,7 @@
 #include <vector>
 
 #include "autotune.hpp"
+#include "cholesky.hpp"
 #include "plain_f64.hpp"
 
 namespace jsvd
@@ -201,10 +201,12 @@ namespace jsvd
         {
             JSVD_HIP_CHECK(hipMalloc(&a64b_, sizeof(double) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&w64_, sizeof(double) * v_elems));
-            JSVD_HIP_CHECK(hipMalloc(&gram64_, sizeof(double) * ga_.n_pad * ga_.n_pad));
-            JSVD_HIP_CHECK(hipMalloc(&racc_, sizeof(double) * ga_.n_pad * ga_.n_pad));
+            // n_pad x n_pad products and the Cholesky factors, padded to the factorisation's 64-row tiles
+            const int64_t n64 = cholesky_padded_size(ga_.n_pad);
+            JSVD_HIP_CHECK(hipMalloc(&gram64_, sizeof(double) * n64 * n64));
+            JSVD_HIP_CHECK(hipMalloc(&racc_, sizeof(double) * n64 * n64));
             JSVD_HIP_CHECK(hipMalloc(&dcol_, sizeof(double) * ga_.n_pad));
-            JSVD_HIP_CHECK(hipMalloc(&chol_info_, sizeof(int)));
+            JSVD_HIP_CHECK(hipMalloc(&chol_info_, sizeof(int) * 3));
             JSVD_HIP_CHECK(hipHostMalloc(&h_sigma_, sizeof(double) * ga_.n_pad, hipHostMallocDefault));
             JSVD_HIP_CHECK(hipMalloc(&a32_, sizeof(float) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&v32_, sizeof(float) * v_elems));
@@ -394,7 +396,7 @@ namespace jsvd
         const rocblas_int m = static_cast<rocblas_int>(ga_.m);
         const rocblas_int n = static_cast<rocblas_int>(ga_.n);
         const rocblas_int lda = static_cast<rocblas_int>(ga_.lda);
-        const rocblas_int ldr = static_cast<rocblas_int>(ga_.n_pad);
+        const rocblas_int ldr = static_cast<rocblas_int>(cholesky_padded_size(ga_.n_pad));
         hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(ga_.n_pad)), dim3(256), 0, stream_, a64_,
                            ga_.lda, ga_.m, ga_.n, dcol_);
         JSVD_HIP_CHECK(hipGetLastError());
@@ -413,14 +415,7 @@ namespace jsvd
                                    static_cast<int64_t>(ldr), static_cast<int64_t>(n), shift);
                 JSVD_HIP_CHECK(hipGetLastError());
             }
-            blas_check(rocsolver_dpotrf(h, rocblas_fill_upper, n, gram64_, ldr, chol_info_), "rocsolver_dpotrf");
-            int status = 0;
-            JSVD_HIP_CHECK(hipMemcpyAsync(&status, chol_info_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-            JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
-            if (status != 0)
-            {
-                return false; // not positive definite: the caller solves A directly
-            }
+            launch_cholesky_upper(gram64_, ldr, n, chol_info_ + pass, stream_);
             blas_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                                      rocblas_diagonal_non_unit, m, n, &one, gram64_, ldr, a64_, lda),
                        "dtrsm B R^-1");
@@ -440,6 +435,13 @@ namespace jsvd
                 JSVD_HIP_CHECK(hipMemcpyAsync(racc_, w64_, sizeof(double) * ldr * n, hipMemcpyDeviceToDevice,
                                               stream_));
             }
+        }
+        int status[3] = {0, 0, 0};
+        JSVD_HIP_CHECK(hipMemcpyAsync(status, chol_info_, sizeof(status), hipMemcpyDeviceToHost, stream_));
+        JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
+        if (status[0] != 0 || status[1] != 0 || status[2] != 0)
+        {
+            return false; // a factor was not positive definite: the caller solves A directly
         }
         // M = (R D)^T into a64b_ with the sweep geometry's padding
         hipLaunchKernelGGL(transpose_scaled_kernel, dim3(grid_of(g_.lda * g_.n_pad)), dim3(256), 0, stream_, racc_,

@@ -70,7 +70,7 @@ namespace jsvd
         double *partial_ = nullptr; // reused as float for the FP32 phase
         double *racc_ = nullptr;    // [n_pad x n_pad] accumulated Cholesky QR factor
         double *dcol_ = nullptr;    // [n_pad] column norms of A (QR column scaling)
-        int *chol_info_ = nullptr;  // [1] potrf status
+        int *chol_info_ = nullptr;  // [3] Cholesky status of the three QR passes
         double *h_sigma_ = nullptr; // [n_pad] pinned copy of sigma (zero check)
         double *q_ = nullptr;
         int *flags_ = nullptr;

@@ -191,6 +191,7 @@ def lib() -> ctypes.CDLL:
         L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_block_eigen.argtypes = [ctypes.c_int32, ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
+        L.jsvd_cholesky_f64.argtypes = [_vp, _i64, _i64, _vp, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp, _vp]
         for name in declared_symbols():
@@ -410,6 +411,20 @@ def offnorm(a_col, lda: int, m: int, n: int, out, stream=None) -> None:
 
 def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) -> None:
     check(lib().jsvd_u_sigma_f64(_ptr(a_col), lda, m, n, eps, _ptr(u), _ptr(sigma), _stream_handle(stream)))
+
+
+def cholesky(a, n: int, info, stream=None) -> None:
+    """A = R^T R in place (jsvd_cholesky_f64) on a float64 CUDA tensor viewed column-major: a is (cols, lda)
+    row-major storage = column-major lda x cols with lda >= n rounded up to 64 and cols >= that too; info is an
+    int32 CUDA tensor of one element."""
+    import torch
+    if a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous() or a.dim() != 2:
+        raise ValueError("a must be a contiguous float64 CUDA matrix")
+    lda = a.shape[1]
+    n64 = (n + 63) // 64 * 64
+    if a.shape[0] < n64:
+        raise ValueError("a needs n rounded up to 64 columns")
+    check(lib().jsvd_cholesky_f64(_ptr(a), lda, n, _ptr(info), _stream_handle(stream)))
 
 
 def block_eigen(gram, eps: float, q, rotations, stream=None) -> None:
