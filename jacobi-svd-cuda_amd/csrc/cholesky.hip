@@ -174,7 +174,10 @@ namespace jsvd
                 for (int v = 0; v < 4; ++v)
                 {
                     const int row = 16 * wave + lane / 16 + 4 * v;
-                    out[static_cast<int64_t>(col) * ld + row] -= acc[t][v];
+                    if (i < j || row <= col) // a diagonal tile keeps its strictly lower triangle
+                    {
+                        out[static_cast<int64_t>(col) * ld + row] -= acc[t][v];
+                    }
                 }
             }
         }
