@@ -2,10 +2,10 @@
 // FP64 matrix, used by the mixed path's Cholesky QR.
 //
 // Right-looking over 64 x 64 tiles, three launches per tile step k:
-//   cholesky_diag_kernel    one workgroup: factors the diagonal tile A_kk in LDS (64 steps of a rank-1 update of
-//                           the trailing rows with the unscaled pivot row, then the row scaled) and writes R_kk
-//   cholesky_panel_kernel   one workgroup per tile column j > k: solves R_kk^T R_kj = A_kj by forward
-//                           substitution in LDS
+//   cholesky_diag_kernel    one wavefront: factors the diagonal tile A_kk with its columns in registers (64
+//                           steps: pivot and scaled row broadcast across lanes, rank-1 update) and writes R_kk
+//   cholesky_panel_kernel   one wavefront per tile column j > k: solves R_kk^T R_kj = A_kj by forward
+//                           substitution, each lane one column in registers
 //   cholesky_update_kernel  one workgroup per trailing tile (i, j), k < i <= j: A_ij -= R_ki^T R_kj on FP64 MFMA
 //                           (v_mfma_f64_16x16x4f64), both operand tiles staged in LDS
 // Only the upper triangle is read and written; the strictly lower triangle keeps whatever it held.
@@ -19,107 +19,93 @@ namespace jsvd
 
         typedef double f64x4c __attribute__((ext_vector_type(4)));
 
-        __global__ __launch_bounds__(256) void cholesky_diag_kernel(double *__restrict__ a, int64_t ld, int k,
-                                                                    int *__restrict__ info)
+        // One wavefront: lane l holds column l of the diagonal tile in registers (a[r] = A[r][l]).  Step c
+        // broadcasts the pivot and the scaled row c (R[c][i] from lane i) and updates every lane's column; the
+        // lanes' strictly lower entries take part in the arithmetic but are never written back.
+        __global__ __launch_bounds__(64) void cholesky_diag_kernel(double *__restrict__ a, int64_t ld, int k,
+                                                                   int *__restrict__ info)
         {
-            __shared__ double d[kT][kT + 1]; // diagonal tile, d[col][row]
-            __shared__ int bad;
-            const int tid = threadIdx.x;
+            const int l = threadIdx.x;
             const int64_t r0 = static_cast<int64_t>(k) * kT;
-            double *akk = a + r0 * ld + r0;
-            for (int e = tid; e < kT * kT; e += 256)
+            double *col = a + (r0 + l) * ld + r0;
+            double x[kT];
+#pragma unroll
+            for (int r = 0; r < kT; ++r)
             {
-                const int c = e / kT, r = e % kT;
-                d[c][r] = akk[static_cast<int64_t>(c) * ld + r];
+                x[r] = col[r];
             }
-            if (tid == 0)
-            {
-                bad = 0;
-            }
-            __syncthreads();
-            // Step c updates rows c+1.. with the unscaled row c (A[i][l] -= A[c][i] A[c][l] / A[c][c]), then
-            // scales row c by 1 / sqrt(A[c][c]).
+            bool bad = false;
+#pragma unroll
             for (int c = 0; c < kT; ++c)
             {
-                double piv = d[c][c];
+                double piv = __shfl(x[c], c, 64);
                 if (!(piv > 0.0))
                 {
-                    if (tid == 0)
-                    {
-                        bad = 1;
-                    }
+                    bad = true;
                     piv = 1.0; // keep the arithmetic finite; the factorisation is reported as failed
                 }
-                const int w = kT - 1 - c; // trailing size
-                for (int e = tid; e < w * w; e += 256)
+                const double r = sqrt(piv);
+                const double u = x[c] / r; // R[c][l] for l > c
+                x[c] = (l == c) ? r : u;
+#pragma unroll
+                for (int i = c + 1; i < kT; ++i)
                 {
-                    const int i = c + 1 + e / w;
-                    const int l = c + 1 + e % w;
-                    if (i <= l)
-                    {
-                        d[l][i] -= d[i][c] * d[l][c] / piv;
-                    }
+                    x[i] = fma(-__shfl(u, i, 64), u, x[i]);
                 }
-                __syncthreads();
-                if (tid >= c && tid < kT)
-                {
-                    const double r = sqrt(piv);
-                    d[tid][c] = (tid == c) ? r : d[tid][c] / r;
-                }
-                __syncthreads();
             }
-            for (int e = tid; e < kT * kT; e += 256)
+#pragma unroll
+            for (int r = 0; r < kT; ++r)
             {
-                const int c = e / kT, r = e % kT;
-                if (r <= c)
+                if (r <= l)
                 {
-                    akk[static_cast<int64_t>(c) * ld + r] = d[c][r];
+                    col[r] = x[r];
                 }
             }
-            if (tid == 0 && bad)
+            if (l == 0 && bad)
             {
                 atomicCAS(info, 0, k * kT + 1);
             }
         }
 
-        // Tile column j = k + 1 + blockIdx.x of block row k: R_kk^T R_kj = A_kj, row by row (Y[r][:] = X[r][:] /
-        // R[r][r], then X[p][:] -= R[r][p] Y[r][:] for p > r).
-        __global__ __launch_bounds__(256) void cholesky_panel_kernel(double *__restrict__ a, int64_t ld, int k)
+        // Tile column j = k + 1 + blockIdx.x of block row k, one wavefront: lane c holds column c of the tile in
+        // registers and solves R_kk^T y = x by forward substitution (y[r] = x[r] / R[r][r], then x[p] -= R[r][p]
+        // y[r] for p > r), R_kk staged in LDS and read as broadcasts.
+        __global__ __launch_bounds__(64) void cholesky_panel_kernel(double *__restrict__ a, int64_t ld, int k)
         {
-            __shared__ double d[kT][kT + 1]; // R_kk, d[col][row] (upper part read)
-            __shared__ double x[kT][kT + 1]; // the panel tile, x[col][row]
+            __shared__ double rt[kT][kT]; // rt[p][r] = R_kk[r][p] (column p of R_kk), upper part read
+            __shared__ double inv[kT];    // 1 / R_kk[r][r]
             const int j = k + 1 + static_cast<int>(blockIdx.x);
-            const int tid = threadIdx.x;
+            const int lane = threadIdx.x;
             const int64_t r0 = static_cast<int64_t>(k) * kT;
             const double *akk = a + r0 * ld + r0;
-            double *akj = a + static_cast<int64_t>(j) * kT * ld + r0;
-            for (int e = tid; e < kT * kT; e += 256)
+            for (int p = 0; p < kT; ++p)
             {
-                const int c = e / kT, r = e % kT;
-                d[c][r] = akk[static_cast<int64_t>(c) * ld + r];
-                x[c][r] = akj[static_cast<int64_t>(c) * ld + r];
+                rt[p][lane] = akk[static_cast<int64_t>(p) * ld + lane];
             }
-            __syncthreads();
+            double *col = a + (static_cast<int64_t>(j) * kT + lane) * ld + r0;
+            double x[kT];
+#pragma unroll
             for (int r = 0; r < kT; ++r)
             {
-                if (tid < kT)
-                {
-                    x[tid][r] /= d[r][r];
-                }
-                __syncthreads();
-                const int w = kT - 1 - r;
-                for (int e = tid; e < w * kT; e += 256)
-                {
-                    const int p = r + 1 + e / kT;
-                    const int col = e % kT;
-                    x[col][p] -= d[p][r] * x[col][r];
-                }
-                __syncthreads();
+                x[r] = col[r];
             }
-            for (int e = tid; e < kT * kT; e += 256)
+            __syncthreads();
+            inv[lane] = 1.0 / rt[lane][lane];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kT; ++r)
             {
-                const int c = e / kT, r = e % kT;
-                akj[static_cast<int64_t>(c) * ld + r] = x[c][r];
+                x[r] *= inv[r];
+#pragma unroll
+                for (int p = r + 1; p < kT; ++p)
+                {
+                    x[p] = fma(-rt[p][r], x[r], x[p]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kT; ++r)
+            {
+                col[r] = x[r];
             }
         }
 
@@ -224,12 +210,12 @@ namespace jsvd
         const int nt = static_cast<int>(n64 / kT);
         for (int k = 0; k < nt; ++k)
         {
-            hipLaunchKernelGGL(cholesky_diag_kernel, dim3(1), dim3(256), 0, stream, a, ld, k, info);
+            hipLaunchKernelGGL(cholesky_diag_kernel, dim3(1), dim3(64), 0, stream, a, ld, k, info);
             JSVD_HIP_CHECK(hipGetLastError());
             const int rest = nt - k - 1;
             if (rest > 0)
             {
-                hipLaunchKernelGGL(cholesky_panel_kernel, dim3(static_cast<unsigned>(rest)), dim3(256), 0, stream, a,
+                hipLaunchKernelGGL(cholesky_panel_kernel, dim3(static_cast<unsigned>(rest)), dim3(64), 0, stream, a,
                                    ld, k);
                 JSVD_HIP_CHECK(hipGetLastError());
                 hipLaunchKernelGGL(cholesky_update_kernel, dim3(static_cast<unsigned>(rest), static_cast<unsigned>(rest)),
