@@ -19,9 +19,18 @@ namespace jsvd
 
         typedef double f64x4c __attribute__((ext_vector_type(4)));
 
-        // One wavefront: lane l holds column l of the diagonal tile in registers (a[r] = A[r][l]).  Step c
-        // broadcasts the pivot and the scaled row c (R[c][i] from lane i) and updates every lane's column; the
-        // lanes' strictly lower entries take part in the arithmetic but are never written back.
+        // v of lane `lane` (a compile-time constant at every call) read into scalar registers.
+        __device__ inline double lane_value(double v, int lane)
+        {
+            const long long bits = __double_as_longlong(v);
+            const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), lane);
+            const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+            return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+        }
+
+        // One wavefront: lane l holds column l of the diagonal tile in registers (x[r] = A[r][l]).  Step c reads
+        // the pivot and the scaled row c (R[c][i] from lane i) with v_readlane and updates every lane's column;
+        // the lanes' strictly lower entries take part in the arithmetic but are never written back.
         __global__ __launch_bounds__(64) void cholesky_diag_kernel(double *__restrict__ a, int64_t ld, int k,
                                                                    int *__restrict__ info)
         {
@@ -38,7 +47,7 @@ namespace jsvd
 #pragma unroll
             for (int c = 0; c < kT; ++c)
             {
-                double piv = __shfl(x[c], c, 64);
+                double piv = lane_value(x[c], c);
                 if (!(piv > 0.0))
                 {
                     bad = true;
@@ -50,7 +59,7 @@ namespace jsvd
 #pragma unroll
                 for (int i = c + 1; i < kT; ++i)
                 {
-                    x[i] = fma(-__shfl(u, i, 64), u, x[i]);
+                    x[i] = fma(-lane_value(u, i), u, x[i]);
                 }
             }
 #pragma unroll
