@@ -319,6 +319,7 @@ typedef struct jsvd_mat_stream_report
     double seconds_output;    /* solver thread: device encode, copy back, append to the output file */
     int32_t sweeps_fp32_last; /* sweeps of the last record */
     int32_t sweeps_fp64_last;
+    int64_t total_sweeps;     /* FP32 + FP64 sweeps over all records */
 } jsvd_mat_stream_report;
 
 /*

@@ -83,7 +83,27 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
         raise RuntimeError("link of libjsvd.so failed")
     shutil.move(tmp, LIB)
     build_cpp_test(verbose)
+    build_cli(verbose)
     return LIB
+
+
+CLI_SRC = os.path.join(PKG, "cli", "main.cpp")
+CLI_BIN = os.path.join(LIBDIR, "jacobi-svd")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The command-line front end (the reference CLI's options and reports), g++ linked to libjsvd.so."""
+    tmp = CLI_BIN + ".tmp"
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CLI_SRC, "-o", tmp,
+           "-L", LIBDIR, "-ljsvd", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("build of the CLI failed")
+    shutil.move(tmp, CLI_BIN)
+    return CLI_BIN
 
 
 CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")

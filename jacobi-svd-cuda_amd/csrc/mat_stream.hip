@@ -337,6 +337,7 @@ namespace jsvd
                     rep.payload_bytes += static_cast<int64_t>(r.count() * kValueBytes);
                     rep.sweeps_fp32_last = info.sweeps_fp32;
                     rep.sweeps_fp64_last = info.sweeps_fp64;
+                    rep.total_sweeps += info.sweeps_fp32 + info.sweeps_fp64;
                 }
                 if (out)
                 {

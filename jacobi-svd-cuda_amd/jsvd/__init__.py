@@ -700,6 +700,7 @@ class MatStreamReport(ctypes.Structure):
         ("seconds_output", ctypes.c_double),
         ("sweeps_fp32_last", ctypes.c_int32),
         ("sweeps_fp64_last", ctypes.c_int32),
+        ("total_sweeps", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
