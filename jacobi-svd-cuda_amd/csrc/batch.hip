@@ -4,8 +4,8 @@
 // matrix alone: the reference algorithm (round-robin pair order, rotation formula and skip test,
 // rotation-free-sweep convergence, sigma/U extraction with the absolute epsilon cutoff) in FP32 or FP64
 // storage and accumulation, with the rotation parameters always in FP64.  Column pairs of a round are spread
-// over the four wavefronts (pair k -> wave k % 4), each pair reduced with wavefront butterflies, so an item's
-// arithmetic does not depend on the workgroup, the queue order or the other items.
+// over the four wavefronts (pair k -> wave k % 4), each pair reduced within its wavefront by DPP row moves, so
+// an item's arithmetic does not depend on the workgroup, the queue order or the other items.
 //
 // A (column-major, leading dimension m rounded up to even) and V (column-major n x n, leading dimension n
 // rounded up to even) live in LDS when they fit in the workgroup's budget, otherwise in the workgroup's
@@ -23,17 +23,6 @@ namespace jsvd
         constexpr int kBatchThreads = 256;
         constexpr int kBatchWaves = kBatchThreads / 64;
         constexpr int kBatchLdsBytes = 76 * 1024; // two workgroups per CU
-
-        template <typename T>
-        __device__ inline T wave_sum_t(T x)
-        {
-#pragma unroll
-            for (int offset = 32; offset > 0; offset >>= 1)
-            {
-                x += __shfl_xor(x, offset, 64);
-            }
-            return x;
-        }
 
         __device__ inline bool item_ok(const jsvd_batch_item &it, const jsvd_batch_buffers &b, int max_rows,
                                        int max_cols)
@@ -106,9 +95,9 @@ namespace jsvd
                             sqq += y * y;
                             spq += x * y;
                         }
-                        spp = wave_sum_t(spp);
-                        sqq = wave_sum_t(sqq);
-                        spq = wave_sum_t(spq);
+                        spp = dev::wave_sum_dpp(spp);
+                        sqq = dev::wave_sum_dpp(sqq);
+                        spq = dev::wave_sum_dpp(spq);
                         double c, sn;
                         if (dev::rotation_params(static_cast<double>(spp), static_cast<double>(sqq),
                                                  static_cast<double>(spq), eps, c, sn))

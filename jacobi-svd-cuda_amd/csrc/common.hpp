@@ -206,6 +206,50 @@ namespace jsvd::dev
         return x;
     }
 
+    // Lane values moved by a DPP pattern (CTRL, ROW_MASK: compile-time), 0 where the pattern reads outside a
+    // row or the row is masked off.  64-bit values move as two 32-bit halves with the same pattern.
+    template <int CTRL, int ROW_MASK>
+    __device__ inline float dpp_move(float v)
+    {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
+    }
+
+    template <int CTRL, int ROW_MASK>
+    __device__ inline double dpp_move(double v)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROW_MASK, 0xF, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xF, true);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+
+    __device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+
+    __device__ inline double lane63(double v)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
+        const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+
+    // Sum over the 64 lanes of a wavefront with DPP row moves instead of LDS-crossbar shuffles: an inclusive
+    // scan inside each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's into row 3
+    // (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31); lane 63 holds the total, which every lane
+    // reads back (v_readlane).  A fixed summation order, so the result does not depend on anything but the
+    // lane values.
+    template <typename T>
+    __device__ inline T wave_sum_dpp(T x)
+    {
+        x += dpp_move<0x111, 0xF>(x); // row_shr:1
+        x += dpp_move<0x112, 0xF>(x); // row_shr:2
+        x += dpp_move<0x114, 0xF>(x); // row_shr:4
+        x += dpp_move<0x118, 0xF>(x); // row_shr:8
+        x += dpp_move<0x142, 0xA>(x); // row_bcast:15 into rows 1 and 3
+        x += dpp_move<0x143, 0xC>(x); // row_bcast:31 into rows 2 and 3
+        return lane63(x);
+    }
+
     __device__ inline double wave_max(double x)
     {
 #pragma unroll
