@@ -246,6 +246,38 @@ int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, i
 int jsvd_batch_solve_async(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
                            const jsvd_batch_buffers *buf, void *done);
 
+/*
+ * Launch shape of the persistent batched kernel (no reference counterpart): threads per workgroup (the
+ * wavefronts sharing one matrix's column pairs) and the LDS bytes each workgroup stages its matrix in.  The
+ * shapes are (256, 77824) (the default), (128, 38912), (256, 38912), (512, 77824), (512, 155648) and
+ * (1024, 155648); (0, 0) selects the default.  Every shape gives bit-identical results; only speed differs.
+ * A batch created for limits that jsvd_batch_autotune has measured in this process starts with that choice.
+ */
+#define JSVD_BATCH_MAX_SHAPES 8
+int jsvd_batch_set_shape(jsvd_batch *batch, int32_t threads, int32_t lds_bytes);
+int jsvd_batch_get_shape(const jsvd_batch *batch, int32_t *threads, int32_t *lds_bytes, int32_t *workgroups);
+
+/* Per-shape timings of one jsvd_batch_autotune call. */
+typedef struct jsvd_batch_tune_report
+{
+    int32_t candidates;                       /* shapes measured */
+    int32_t chosen;                           /* index of the fastest, now set on the batch */
+    int32_t threads[JSVD_BATCH_MAX_SHAPES];
+    int32_t lds_bytes[JSVD_BATCH_MAX_SHAPES];
+    int32_t workgroups[JSVD_BATCH_MAX_SHAPES]; /* resident workgroups of the launch */
+    int32_t reserved;
+    double ms[JSVD_BATCH_MAX_SHAPES];         /* mean time of one solve of the whole batch */
+} jsvd_batch_tune_report;
+
+/*
+ * Batch autotuner (no reference counterpart): solves the given batch `repetitions` times (after one warm-up)
+ * with every launch shape, keeps the fastest on `batch` and records it for its limits (max_rows, max_columns),
+ * so batches created later with the same limits (the request loop's) start with it.  The buffers receive the
+ * batch's results, as after jsvd_batch_solve_device.  Synchronous.
+ */
+int jsvd_batch_autotune(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
+                        const jsvd_batch_buffers *buf, int32_t repetitions, jsvd_batch_tune_report *report);
+
 /* ---------------------------------------------------------------------------------------------------
  * Request loop (continuous batching).  No reference counterpart (the reference pipeline solves a file of
  * cases with one thread per case, pipeline.cpp:754-863).  Requests are admitted against an HBM budget;

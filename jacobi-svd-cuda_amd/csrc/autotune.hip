@@ -270,6 +270,14 @@ namespace jsvd
         }
     } // namespace
 
+    void store_batch_shape(int64_t max_rows, int64_t max_columns, int shape)
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        Tuning &e = g_table[{max_rows, max_columns}];
+        e.batch_shape = shape;
+        e.has_batch = true;
+    }
+
     Tuning run_autotune(int64_t m, int64_t n, int repetitions, hipStream_t stream, jsvd_tune_report *report)
     {
         validate_shape(m, n);

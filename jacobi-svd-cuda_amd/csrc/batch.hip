@@ -1,28 +1,27 @@
 // batch.hip -- persistent batched one-sided Jacobi: many small matrices of different shapes in one launch.
 //
-// Each 256-thread workgroup repeatedly takes the next item index from a device counter and solves that
-// matrix alone: the reference algorithm (round-robin pair order, rotation formula and skip test,
-// rotation-free-sweep convergence, sigma/U extraction with the absolute epsilon cutoff) in FP32 or FP64
-// storage and accumulation, with the rotation parameters always in FP64.  Column pairs of a round are spread
-// over the four wavefronts (pair k -> wave k % 4), each pair reduced within its wavefront by DPP row moves, so
-// an item's arithmetic does not depend on the workgroup, the queue order or the other items.
+// Each workgroup repeatedly takes the next item index from a device counter and solves that matrix alone: the
+// reference algorithm (round-robin pair order, rotation formula and skip test, rotation-free-sweep
+// convergence, sigma/U extraction with the absolute epsilon cutoff) in FP32 or FP64 storage and accumulation,
+// with the rotation parameters always in FP64.  Column pairs of a round are spread over the workgroup's
+// wavefronts (pair k -> wave k % waves), each pair reduced within its wavefront by DPP row moves, so an item's
+// arithmetic does not depend on the workgroup size, the queue order or the other items.
 //
 // A (column-major, leading dimension m rounded up to even) and V (column-major n x n, leading dimension n
 // rounded up to even) live in LDS when they fit in the workgroup's budget, otherwise in the workgroup's
-// slice of a global scratch area (L2/MALL resident).
+// slice of a global scratch area (L2/MALL resident).  The workgroup size and LDS budget form the launch
+// shape, chosen among a fixed table by measurement (jsvd_batch_autotune).
 #include <algorithm>
 #include <cstring>
 #include <memory>
 
+#include "autotune.hpp"
 #include "common.hpp"
 
 namespace jsvd
 {
     namespace
     {
-        constexpr int kBatchThreads = 256;
-        constexpr int kBatchWaves = kBatchThreads / 64;
-        constexpr int kBatchLdsBytes = 76 * 1024; // two workgroups per CU
 
         __device__ inline bool item_ok(const jsvd_batch_item &it, const jsvd_batch_buffers &b, int max_rows,
                                        int max_cols)
@@ -39,10 +38,11 @@ namespace jsvd
                    it.v_offset >= 0 && it.v_offset + n * n <= b.v_len;
         }
 
-        template <typename T>
+        template <typename T, int THREADS>
         __device__ void solve_item(const jsvd_batch_item &it, const jsvd_batch_buffers &b, T *a, T *v, int *sweep_rot,
                                    int64_t index)
         {
+            constexpr int kWaves = THREADS / 64;
             const int m = it.rows;
             const int n = it.columns;
             const int lda = m + (m & 1);
@@ -53,13 +53,13 @@ namespace jsvd
             const double eps = it.tolerance;
             // A row-major (global, FP64) -> column-major T; V = I.
             const double *src = b.a + it.a_offset;
-            for (int e = tid; e < m * n; e += kBatchThreads)
+            for (int e = tid; e < m * n; e += THREADS)
             {
                 const int i = e / n;
                 const int j = e % n;
                 a[j * lda + i] = static_cast<T>(src[e]);
             }
-            for (int e = tid; e < n * ldv; e += kBatchThreads)
+            for (int e = tid; e < n * ldv; e += THREADS)
             {
                 v[e] = (e / ldv == e % ldv) ? T(1) : T(0);
             }
@@ -76,7 +76,7 @@ namespace jsvd
                 __syncthreads();
                 for (int r = 0; r < rounds; ++r)
                 {
-                    for (int k = wave; k < N / 2; k += kBatchWaves)
+                    for (int k = wave; k < N / 2; k += kWaves)
                     {
                         const int p = dev::circle_at_fast(k, r, N);
                         const int q = dev::circle_at_fast(N - 1 - k, r, N);
@@ -140,7 +140,7 @@ namespace jsvd
             double *u_out = b.u + it.u_offset;
             double *s_out = b.sigma + it.sigma_offset;
             double *v_out = b.v + it.v_offset;
-            for (int j = wave; j < n; j += kBatchWaves)
+            for (int j = wave; j < n; j += kWaves)
             {
                 const T *cj = a + j * lda;
                 double acc = 0.0;
@@ -160,7 +160,7 @@ namespace jsvd
                     u_out[static_cast<int64_t>(i) * n + j] = (sj > eps) ? (static_cast<double>(cj[i]) / sj) : 0.0;
                 }
             }
-            for (int e = tid; e < n * n; e += kBatchThreads)
+            for (int e = tid; e < n * n; e += THREADS)
             {
                 const int i = e / n;
                 const int j = e % n;
@@ -173,13 +173,14 @@ namespace jsvd
             __syncthreads(); // LDS / scratch reuse by the next item
         }
 
-        __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const jsvd_batch_item *__restrict__ items,
+        template <int THREADS, int LDS_BYTES>
+        __global__ __launch_bounds__(THREADS) void batch_kernel(const jsvd_batch_item *__restrict__ items,
                                                                      int64_t count, jsvd_batch_buffers b,
                                                                      unsigned long long *__restrict__ counter,
                                                                      double *__restrict__ scratch,
                                                                      int64_t scratch_per_wg, int max_rows, int max_cols)
         {
-            __shared__ __align__(16) unsigned char lds[kBatchLdsBytes];
+            __shared__ __align__(16) unsigned char lds[LDS_BYTES];
             __shared__ int sweep_rot;
             __shared__ long long next;
             double *my_scratch = scratch + static_cast<int64_t>(blockIdx.x) * scratch_per_wg;
@@ -211,24 +212,71 @@ namespace jsvd
                 const int64_t v_elems = static_cast<int64_t>(n + (n & 1)) * n;
                 if (it.precision == JSVD_PRECISION_F32)
                 {
-                    const int64_t budget = kBatchLdsBytes / sizeof(float);
+                    const int64_t budget = LDS_BYTES / sizeof(float);
                     float *lds_f = reinterpret_cast<float *>(lds);
                     float *scr_f = reinterpret_cast<float *>(my_scratch);
                     float *a = (a_elems <= budget) ? lds_f : scr_f;
                     const int64_t used = (a_elems <= budget) ? a_elems : 0;
                     float *v = (used + v_elems <= budget) ? lds_f + used : scr_f + a_elems;
-                    solve_item<float>(it, b, a, v, &sweep_rot, idx);
+                    solve_item<float, THREADS>(it, b, a, v, &sweep_rot, idx);
                 }
                 else
                 {
-                    const int64_t budget = kBatchLdsBytes / sizeof(double);
+                    const int64_t budget = LDS_BYTES / sizeof(double);
                     double *lds_d = reinterpret_cast<double *>(lds);
                     double *a = (a_elems <= budget) ? lds_d : my_scratch;
                     const int64_t used = (a_elems <= budget) ? a_elems : 0;
                     double *v = (used + v_elems <= budget) ? lds_d + used : my_scratch + a_elems;
-                    solve_item<double>(it, b, a, v, &sweep_rot, idx);
+                    solve_item<double, THREADS>(it, b, a, v, &sweep_rot, idx);
                 }
             }
+        }
+    } // namespace
+} // namespace jsvd
+
+namespace jsvd
+{
+    namespace
+    {
+        using BatchKernel = void (*)(const jsvd_batch_item *, int64_t, jsvd_batch_buffers, unsigned long long *,
+                                     double *, int64_t, int, int);
+
+        // The launch shapes the batch autotuner chooses among: threads per workgroup (the wavefronts that share
+        // one matrix's column pairs) and the LDS bytes each workgroup stages its matrix in (what does not fit
+        // goes to the workgroup's scratch slice).  Every shape runs the same per-item arithmetic, so an item's
+        // result is bit-identical whichever shape solves it.
+        struct BatchShape
+        {
+            int threads;
+            int lds_bytes;
+            BatchKernel kernel;
+        };
+        constexpr int kKiB = 1024;
+        const BatchShape kShapes[] = {
+            {256, 76 * kKiB, batch_kernel<256, 76 * kKiB>},     // default: 2 workgroups x 4 waves per CU
+            {128, 38 * kKiB, batch_kernel<128, 38 * kKiB>},     // 4 x 2 waves, more items in flight
+            {256, 38 * kKiB, batch_kernel<256, 38 * kKiB>},     // 4 x 4 waves
+            {512, 76 * kKiB, batch_kernel<512, 76 * kKiB>},     // 2 x 8 waves
+            {512, 152 * kKiB, batch_kernel<512, 152 * kKiB>},   // 1 x 8 waves, most matrices LDS-resident
+            {1024, 152 * kKiB, batch_kernel<1024, 152 * kKiB>}, // 1 x 16 waves
+        };
+        constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
+        static_assert(kShapeCount <= JSVD_BATCH_MAX_SHAPES, "report arrays hold every shape");
+
+        int find_shape(int threads, int lds_bytes)
+        {
+            if (threads == 0 && lds_bytes == 0)
+            {
+                return 0;
+            }
+            for (int i = 0; i < kShapeCount; ++i)
+            {
+                if (kShapes[i].threads == threads && kShapes[i].lds_bytes == lds_bytes)
+                {
+                    return i;
+                }
+            }
+            return -1;
         }
     } // namespace
 } // namespace jsvd
@@ -237,8 +285,11 @@ struct jsvd_batch
 {
     int64_t max_rows = 0;
     int64_t max_cols = 0;
-    int grid = 0;
+    int shape = 0;             // index into the shape table
+    int grid = 0;              // resident workgroups of the current shape
+    int cus = 1;
     int64_t scratch_per_wg = 0; // doubles
+    int64_t scratch_wgs = 0;    // workgroup slices the scratch area holds
     hipStream_t stream = nullptr;
     bool own_stream = false;
     double *scratch = nullptr;
@@ -258,6 +309,27 @@ struct jsvd_batch
         }
     }
 
+    // Selects launch shape `index`: one resident wave of workgroups (occupancy x CUs) and a scratch slice each.
+    void set_shape(int index)
+    {
+        const jsvd::BatchShape &sh = jsvd::kShapes[index];
+        int per_cu = 0;
+        JSVD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(sh.kernel), sh.threads, 0));
+        const int g = std::max(per_cu, 1) * cus;
+        if (g > scratch_wgs)
+        {
+            JSVD_HIP_CHECK(hipStreamSynchronize(stream)); // launches in flight may still use the old area
+            JSVD_HIP_CHECK(hipFree(scratch));
+            scratch = nullptr;
+            scratch_wgs = 0;
+            JSVD_HIP_CHECK(hipMalloc(&scratch, sizeof(double) * scratch_per_wg * g));
+            scratch_wgs = g;
+        }
+        shape = index;
+        grid = g;
+    }
+
     void enqueue(const jsvd_batch_item *d_items, int64_t count, const jsvd_batch_buffers &buf)
     {
         if (count == 0)
@@ -266,8 +338,9 @@ struct jsvd_batch
         }
         JSVD_HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(unsigned long long), stream));
         const int blocks = static_cast<int>(std::min<int64_t>(grid, count));
-        hipLaunchKernelGGL(jsvd::batch_kernel, dim3(blocks), dim3(jsvd::kBatchThreads), 0, stream, d_items, count, buf,
-                           counter, scratch, scratch_per_wg, static_cast<int>(max_rows), static_cast<int>(max_cols));
+        const jsvd::BatchShape &sh = jsvd::kShapes[shape];
+        hipLaunchKernelGGL(sh.kernel, dim3(blocks), dim3(sh.threads), 0, stream, d_items, count, buf, counter, scratch,
+                           scratch_per_wg, static_cast<int>(max_rows), static_cast<int>(max_cols));
         JSVD_HIP_CHECK(hipGetLastError());
     }
 };
@@ -308,7 +381,7 @@ int jsvd_batch_create(int64_t max_rows, int64_t max_columns, void *stream, jsvd_
         JSVD_HIP_CHECK(hipGetDevice(&device));
         int cus = 0;
         JSVD_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-        b->grid = 2 * std::max(cus, 1); // LDS budget admits two workgroups per CU
+        b->cus = std::max(cus, 1);
         b->scratch_per_wg = (max_rows + 1) * max_columns + (max_columns + 1) * max_columns;
         if (stream)
         {
@@ -319,8 +392,10 @@ int jsvd_batch_create(int64_t max_rows, int64_t max_columns, void *stream, jsvd_
             JSVD_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             b->own_stream = true;
         }
-        JSVD_HIP_CHECK(hipMalloc(&b->scratch, sizeof(double) * b->scratch_per_wg * b->grid));
         JSVD_HIP_CHECK(hipMalloc(&b->counter, sizeof(unsigned long long)));
+        jsvd::Tuning t;
+        const bool tuned = jsvd::lookup_tuning(max_rows, max_columns, t) && t.has_batch;
+        b->set_shape(tuned ? t.batch_shape : 0);
         *batch = b.release();
     });
 }
@@ -360,6 +435,111 @@ int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, i
         return st;
     }
     return jsvd::guarded([&] { JSVD_HIP_CHECK(hipStreamSynchronize(batch->stream)); });
+}
+
+int jsvd_batch_set_shape(jsvd_batch *batch, int32_t threads, int32_t lds_bytes)
+{
+    return jsvd::guarded([&] {
+        if (!batch)
+        {
+            jsvd::throw_invalid("batch must not be null.");
+        }
+        const int index = jsvd::find_shape(threads, lds_bytes);
+        if (index < 0)
+        {
+            jsvd::throw_invalid("unknown batch launch shape (threads, lds_bytes).");
+        }
+        batch->set_shape(index);
+    });
+}
+
+int jsvd_batch_get_shape(const jsvd_batch *batch, int32_t *threads, int32_t *lds_bytes, int32_t *workgroups)
+{
+    return jsvd::guarded([&] {
+        if (!batch)
+        {
+            jsvd::throw_invalid("batch must not be null.");
+        }
+        const jsvd::BatchShape &sh = jsvd::kShapes[batch->shape];
+        if (threads)
+        {
+            *threads = sh.threads;
+        }
+        if (lds_bytes)
+        {
+            *lds_bytes = sh.lds_bytes;
+        }
+        if (workgroups)
+        {
+            *workgroups = batch->grid;
+        }
+    });
+}
+
+int jsvd_batch_autotune(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
+                        const jsvd_batch_buffers *buf, int32_t repetitions, jsvd_batch_tune_report *report)
+{
+    return jsvd::guarded([&] {
+        if (!batch)
+        {
+            jsvd::throw_invalid("batch must not be null.");
+        }
+        check_buffers(buf, count);
+        if (count < 1 || !d_items)
+        {
+            jsvd::throw_invalid("autotuning needs at least one item.");
+        }
+        if (repetitions < 1)
+        {
+            jsvd::throw_invalid("repetitions must be positive.");
+        }
+        jsvd_batch_tune_report rep{};
+        rep.candidates = jsvd::kShapeCount;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        JSVD_HIP_CHECK(hipEventCreate(&t0));
+        JSVD_HIP_CHECK(hipEventCreate(&t1));
+        int best = 0;
+        try
+        {
+            for (int i = 0; i < jsvd::kShapeCount; ++i)
+            {
+                batch->set_shape(i);
+                batch->enqueue(d_items, count, *buf); // warm-up (first launch of this shape)
+                JSVD_HIP_CHECK(hipEventRecord(t0, batch->stream));
+                for (int r = 0; r < repetitions; ++r)
+                {
+                    batch->enqueue(d_items, count, *buf);
+                }
+                JSVD_HIP_CHECK(hipEventRecord(t1, batch->stream));
+                JSVD_HIP_CHECK(hipEventSynchronize(t1));
+                float ms = 0.0f;
+                JSVD_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                rep.threads[i] = jsvd::kShapes[i].threads;
+                rep.lds_bytes[i] = jsvd::kShapes[i].lds_bytes;
+                rep.workgroups[i] = batch->grid;
+                rep.ms[i] = static_cast<double>(ms) / repetitions;
+                if (rep.ms[i] < rep.ms[best])
+                {
+                    best = i;
+                }
+            }
+        }
+        catch (...)
+        {
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            throw;
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        batch->set_shape(best);
+        jsvd::store_batch_shape(batch->max_rows, batch->max_cols, best);
+        rep.chosen = best;
+        if (report)
+        {
+            *report = rep;
+        }
+    });
 }
 
 } // extern "C"

@@ -473,7 +473,35 @@ def _bind_batch(L):
     L.jsvd_batch_destroy.argtypes = [_vp]
     L.jsvd_batch_solve_device.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers)]
     L.jsvd_batch_solve_async.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers), _vp]
+    L.jsvd_batch_set_shape.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
+    L.jsvd_batch_get_shape.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32)]
+    L.jsvd_batch_autotune.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers), ctypes.c_int32,
+                                      ctypes.POINTER(BatchTuneReport)]
     L._batch_bound = True
+
+
+BATCH_MAX_SHAPES = 8
+
+# (threads per workgroup, LDS bytes per workgroup) of the batched kernel's launch shapes; the first is the default
+BATCH_SHAPES = ((256, 76 * 1024), (128, 38 * 1024), (256, 38 * 1024), (512, 76 * 1024), (512, 152 * 1024),
+                (1024, 152 * 1024))
+
+
+class BatchTuneReport(ctypes.Structure):
+    """jsvd_batch_tune_report (include/jsvd.h)."""
+    _fields_ = [
+        ("candidates", ctypes.c_int32), ("chosen", ctypes.c_int32),
+        ("threads", ctypes.c_int32 * BATCH_MAX_SHAPES), ("lds_bytes", ctypes.c_int32 * BATCH_MAX_SHAPES),
+        ("workgroups", ctypes.c_int32 * BATCH_MAX_SHAPES), ("reserved", ctypes.c_int32),
+        ("ms", ctypes.c_double * BATCH_MAX_SHAPES),
+    ]
+
+    def as_dict(self) -> dict:
+        k = self.candidates
+        return {"chosen": self.chosen,
+                "shapes": [{"threads": self.threads[i], "lds_bytes": self.lds_bytes[i],
+                            "workgroups": self.workgroups[i], "ms": self.ms[i]} for i in range(k)]}
 
 
 class BatchPacked:
@@ -536,6 +564,28 @@ class Batch:
         buf = BatchBuffers(d_a.data_ptr(), d_u.data_ptr(), d_s.data_ptr(), d_v.data_ptr(), d_sweeps.data_ptr(),
                            d_a.numel(), d_u.numel(), d_s.numel(), d_v.numel())
         check(lib().jsvd_batch_solve_device(self._h, d_items.data_ptr(), int(count), ctypes.byref(buf)))
+
+    def set_shape(self, threads: int = 0, lds_bytes: int = 0):
+        """Launch shape (threads per workgroup, LDS bytes per workgroup), one of BATCH_SHAPES; (0, 0) = default."""
+        check(lib().jsvd_batch_set_shape(self._h, int(threads), int(lds_bytes)))
+
+    def shape(self):
+        """(threads, lds_bytes, resident workgroups) of the current launch shape."""
+        t, l, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().jsvd_batch_get_shape(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(w)))
+        return t.value, l.value, w.value
+
+    def autotune(self, d_items, count, d_a, d_u, d_s, d_v, d_sweeps, repetitions: int = 1) -> BatchTuneReport:
+        """Times every launch shape on this batch (device tensors as for solve_device, which it also fulfils),
+        keeps the fastest and records it for batches with the same limits."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        buf = BatchBuffers(d_a.data_ptr(), d_u.data_ptr(), d_s.data_ptr(), d_v.data_ptr(), d_sweeps.data_ptr(),
+                           d_a.numel(), d_u.numel(), d_s.numel(), d_v.numel())
+        rep = BatchTuneReport()
+        check(lib().jsvd_batch_autotune(self._h, d_items.data_ptr(), int(count), ctypes.byref(buf),
+                                        int(repetitions), ctypes.byref(rep)))
+        return rep
 
     def solve(self, mats, precisions, tolerances, max_sweeps=60, order=None):
         """Host convenience: pack, upload, solve in one launch, download.  Returns (u list, s list, v list,

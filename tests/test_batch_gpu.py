@@ -88,3 +88,83 @@ def test_batch_edge_shapes_and_rejection(torch_cuda):
     us, ss, vs, sw = b.solve([mats[2], big, mats[5]], [0, 0, 0], [1e-14] * 3, max_sweeps=60)
     assert sw[1] == -1 and sw[0] > 0 and sw[2] > 0
     b.close()
+
+
+def _packed_device(torch, mats, prec, tol, max_sweeps=60):
+    pk = jsvd.BatchPacked(mats, prec, tol, max_sweeps)
+    d_items = torch.from_numpy(pk.items[pk.cost_order()].view(np.uint8).copy()).to("cuda")
+    d = [torch.from_numpy(pk.a).to("cuda")]
+    d += [torch.zeros(max(n, 1), dtype=torch.float64, device="cuda") for n in pk.lens[1:]]
+    d.append(torch.zeros(len(mats), dtype=torch.int32, device="cuda"))
+    return d_items, d
+
+
+def _shape_mix():
+    mats, prec, tol = _mix(40, 5)
+    rng = np.random.default_rng(9)
+    for p, t in ((jsvd.PRECISION_F64, 1e-14), (jsvd.PRECISION_F32, 1e-6)):
+        for shape in ((256, 128), (256, 64), (100, 100), (9, 4)):
+            mats.append(rng.standard_normal(shape))
+            prec.append(p)
+            tol.append(t)
+    return mats, prec, tol
+
+
+def test_batch_shapes_bit_identical(torch_cuda):
+    """Every launch shape (workgroup size, LDS budget) gives the same bits: per-item arithmetic is fixed."""
+    torch = torch_cuda
+    mats, prec, tol = _shape_mix()
+    d_items, d = _packed_device(torch, mats, prec, tol)
+    b = jsvd.Batch(256, 128)
+    assert b.shape()[:2] == jsvd.BATCH_SHAPES[0]
+    ref = None
+    for threads, lds in jsvd.BATCH_SHAPES:
+        b.set_shape(threads, lds)
+        t, l, w = b.shape()
+        assert (t, l) == (threads, lds) and w >= 1
+        for x in d[1:]:
+            x.zero_()
+        b.solve_device(d_items, len(mats), *d)
+        out = [x.cpu().numpy().copy() for x in d[1:]]
+        assert np.all(out[-1] >= 1), (threads, lds)
+        if ref is None:
+            ref = out
+        else:
+            for x, y in zip(out, ref):
+                assert np.array_equal(x, y), (threads, lds)
+    with pytest.raises(ValueError):
+        b.set_shape(192, 1024)
+    b.close()
+
+
+def test_batch_autotune_keeps_fastest_and_results(torch_cuda):
+    torch = torch_cuda
+    mats, prec, tol = _shape_mix()
+    d_items, d = _packed_device(torch, mats, prec, tol)
+    jsvd.autotune_clear()
+    b = jsvd.Batch(256, 128)
+    b.solve_device(d_items, len(mats), *d)
+    ref = [x.cpu().numpy().copy() for x in d[1:]]
+    for x in d[1:]:
+        x.zero_()
+    rep = b.autotune(d_items, len(mats), *d, repetitions=1)
+    got = [x.cpu().numpy().copy() for x in d[1:]]
+    for x, y in zip(got, ref):
+        assert np.array_equal(x, y)
+    r = rep.as_dict()
+    assert rep.candidates == len(jsvd.BATCH_SHAPES)
+    ms = [s["ms"] for s in r["shapes"]]
+    assert all(v > 0 for v in ms) and ms[rep.chosen] == min(ms)
+    assert [(s["threads"], s["lds_bytes"]) for s in r["shapes"]] == list(jsvd.BATCH_SHAPES)
+    chosen = jsvd.BATCH_SHAPES[rep.chosen]
+    assert b.shape()[:2] == chosen
+    # batches created later with the same limits start from the measured shape; others keep the default
+    b2 = jsvd.Batch(256, 128)
+    assert b2.shape()[:2] == chosen
+    b3 = jsvd.Batch(200, 100)
+    assert b3.shape()[:2] == jsvd.BATCH_SHAPES[0]
+    jsvd.autotune_clear()
+    b4 = jsvd.Batch(256, 128)
+    assert b4.shape()[:2] == jsvd.BATCH_SHAPES[0]
+    for x in (b, b2, b3, b4):
+        x.close()
