@@ -10,6 +10,9 @@ the plain path's fused round kernel (both timed with HIP events on the stream th
 (the oracle's C restatement of the reference algorithm, OpenMP over the pairs of a round, on a bounded sample
 of sweeps).
 
+Beside it: the streamed *.mat file leg, a batched leg (20,000 uneven matrices in one persistent launch, launch
+shape from the batch autotuner) and a Poisson serving leg (continuous batching, p50/p99 latency).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tall|plain] [--no-cpu-baseline]
 """
 from __future__ import annotations
@@ -160,6 +163,66 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
             "rotated_pairs": rotated, "pairs": pairs}
 
 
+def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
+    """Batched persistent launch over `count` uneven matrices (m ~ U{8..256}, n ~ U{4..min(m,128)}, entries
+    N(0,1) x 10^U(-3,3), alternate items FP32 tol 1e-6 / FP64 tol 1e-14), inputs resident in HBM, queue in cost
+    order.  The launch shape is measured by jsvd_batch_autotune first unless autotuning is off.  Accuracy of a
+    sample of items against LAPACK (numpy.linalg.svd): FP64 items sigma relative error, FP32 items sigma error
+    relative to sigma_max."""
+    from jsvd import workloads
+    items = workloads.batch_mixed(count)
+    mats = [a for a, _, _ in items]
+    prec = [jsvd.PRECISION_F32 if p == "f32" else jsvd.PRECISION_F64 for _, p, _ in items]
+    tol = [t for _, _, t in items]
+    pk = jsvd.BatchPacked(mats, prec, tol, 60)
+    order = pk.cost_order()
+    d_items = torch.from_numpy(pk.items[order].view(np.uint8).copy()).to("cuda")
+    d = [torch.from_numpy(pk.a).to("cuda")]
+    d += [torch.zeros(max(x, 1), dtype=torch.float64, device="cuda") for x in pk.lens[1:]]
+    d.append(torch.zeros(count, dtype=torch.int32, device="cuda"))
+    b = jsvd.Batch(256, 128)
+    try:
+        tune = None
+        if autotune:
+            tune = b.autotune(d_items, count, *d, repetitions=1).as_dict()
+        else:
+            b.solve_device(d_items, count, *d)
+        times = []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.solve_device(d_items, count, *d)
+            times.append(time.perf_counter() - t0)
+        shape = b.shape()
+    finally:
+        b.close()
+    s_all = d[3].cpu().numpy()
+    sweeps = d[5].cpu().numpy()
+    worst64 = worst32 = 0.0
+    for q in range(min(checked, count)):
+        it = pk.items[q]
+        so, nn = int(it["sigma_offset"]), int(it["columns"])
+        ref = np.sort(np.linalg.svd(mats[q], compute_uv=False))[::-1]
+        got = np.sort(s_all[so:so + nn])[::-1]
+        if prec[q] == jsvd.PRECISION_F64:
+            worst64 = max(worst64, float(np.max(np.abs(got - ref) / ref)))
+        else:
+            worst32 = max(worst32, float(np.max(np.abs(got - ref)) / ref[0]))
+    best = min(times)
+    return {
+        "workload": f"{count} matrices, m ~ U{{8..256}}, n ~ U{{4..min(m,128)}}, N(0,1) x 10^U(-3,3), "
+                    "alternate FP32 tol 1e-6 / FP64 tol 1e-14, one persistent launch, cost-ordered queue",
+        "solves_per_s": count / best,
+        "ms_per_batch": best * 1e3,
+        "shape": {"threads": shape[0], "lds_bytes": shape[1], "workgroups": shape[2]},
+        "autotune": tune,
+        "sweeps_mean": float(sweeps.mean()),
+        "rejected": int(np.sum(sweeps < 0)),
+        "accuracy_sample": {"items": min(checked, count), "f64_sigma_relerr_max": worst64,
+                            "f32_sigma_err_rel_sigma_max": worst32},
+    }
+
+
 def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=4):
     """The tall workload as a *.mat file of `records` copies, solved by the streamed file path (mmap -> pinned
     double buffer -> HBM on a copy stream -> device decode, overlapping the next record's load with the current
@@ -290,6 +353,8 @@ def main() -> None:
                     help="block width of the mixed path (0: the autotuner's choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in launch parameters (no jsvd.autotune before the timed legs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-matrices leg")
+    ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving leg")
     ap.add_argument("--serving-rate", type=float, default=1000.0)
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -348,6 +413,12 @@ def main() -> None:
         log("[bench] streamed *.mat file path (mixed)")
         file_stream = file_stream_leg(jsvd, a_host, cfg, mixed["sigma"])
         log(f"[bench] file stream: {file_stream}")
+
+    batch = None
+    if not args.no_batch:
+        log(f"[bench] batched leg: {args.batch_count} uneven matrices in one persistent launch")
+        batch = batch_leg(torch, jsvd, args.batch_count, not args.no_autotune)
+        log(f"[bench] batched: {batch}")
 
     serving = None
     if not args.no_serving:
@@ -418,6 +489,7 @@ def main() -> None:
         "plain_f64": plain,
         "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
         "file_stream_mixed": file_stream,
+        "batched": batch,
         "serving": serving,
         "autotune": tune,
         "roofline": roofline,

@@ -38,6 +38,90 @@ namespace jsvd
                    it.v_offset >= 0 && it.v_offset + n * n <= b.v_len;
         }
 
+        // Gram entries, rotation parameters (FP64) and Givens apply to A and V for P disjoint column pairs
+        // (ps[t], qs[t]) of one round, by one wavefront: lane l owns rows l, l + 64, ...; each pair reduced by DPP.
+        template <typename T, int P>
+        __device__ inline void rotate_pairs(T *a, T *v, int lda, int ldv, int m, int n, const int (&ps)[P],
+                                            const int (&qs)[P], double eps, int lane, int *sweep_rot)
+        {
+            T spp[P], sqq[P], spq[P];
+#pragma unroll
+            for (int t = 0; t < P; ++t)
+            {
+                spp[t] = 0;
+                sqq[t] = 0;
+                spq[t] = 0;
+            }
+            for (int i = lane; i < m; i += 64)
+            {
+#pragma unroll
+                for (int t = 0; t < P; ++t)
+                {
+                    const T x = a[ps[t] * lda + i];
+                    const T y = a[qs[t] * lda + i];
+                    spp[t] += x * x;
+                    sqq[t] += y * y;
+                    spq[t] += x * y;
+                }
+            }
+            bool go[P];
+            T ct[P], st[P];
+            int rotated = 0;
+#pragma unroll
+            for (int t = 0; t < P; ++t)
+            {
+                spp[t] = dev::wave_sum_dpp(spp[t]);
+                sqq[t] = dev::wave_sum_dpp(sqq[t]);
+                spq[t] = dev::wave_sum_dpp(spq[t]);
+                double c = 1.0, sn = 0.0;
+                go[t] = dev::rotation_params(static_cast<double>(spp[t]), static_cast<double>(sqq[t]),
+                                             static_cast<double>(spq[t]), eps, c, sn);
+                ct[t] = static_cast<T>(c);
+                st[t] = static_cast<T>(sn);
+                rotated += go[t] ? 1 : 0;
+            }
+            if (rotated == 0)
+            {
+                return;
+            }
+            for (int i = lane; i < m; i += 64)
+            {
+#pragma unroll
+                for (int t = 0; t < P; ++t)
+                {
+                    if (go[t])
+                    {
+                        T *cp = a + ps[t] * lda;
+                        T *cq = a + qs[t] * lda;
+                        const T x = cp[i];
+                        const T y = cq[i];
+                        cp[i] = ct[t] * x - st[t] * y;
+                        cq[i] = st[t] * x + ct[t] * y;
+                    }
+                }
+            }
+            for (int i = lane; i < n; i += 64)
+            {
+#pragma unroll
+                for (int t = 0; t < P; ++t)
+                {
+                    if (go[t])
+                    {
+                        T *vp = v + ps[t] * ldv;
+                        T *vq = v + qs[t] * ldv;
+                        const T x = vp[i];
+                        const T y = vq[i];
+                        vp[i] = ct[t] * x - st[t] * y;
+                        vq[i] = st[t] * x + ct[t] * y;
+                    }
+                }
+            }
+            if (lane == 0)
+            {
+                atomicAdd(sweep_rot, rotated);
+            }
+        }
+
         template <typename T, int THREADS>
         __device__ void solve_item(const jsvd_batch_item &it, const jsvd_batch_buffers &b, T *a, T *v, int *sweep_rot,
                                    int64_t index)
@@ -76,54 +160,50 @@ namespace jsvd
                 __syncthreads();
                 for (int r = 0; r < rounds; ++r)
                 {
-                    for (int k = wave; k < N / 2; k += kWaves)
+                    // Two pairs per step (k and k + waves) with their loads, reductions and rotation parameters
+                    // interleaved; every pair's own arithmetic is the same as alone.
+                    for (int k = wave; k < N / 2; k += 2 * kWaves)
                     {
-                        const int p = dev::circle_at_fast(k, r, N);
-                        const int q = dev::circle_at_fast(N - 1 - k, r, N);
-                        if (p >= n || q >= n)
+                        int p0 = -1, q0 = -1, p1 = -1, q1 = -1;
                         {
-                            continue;
+                            const int p = dev::circle_at_fast(k, r, N);
+                            const int q = dev::circle_at_fast(N - 1 - k, r, N);
+                            if (p < n && q < n)
+                            {
+                                p0 = p;
+                                q0 = q;
+                            }
                         }
-                        T *cp = a + p * lda;
-                        T *cq = a + q * lda;
-                        T spp = 0, sqq = 0, spq = 0;
-                        for (int i = lane; i < m; i += 64)
+                        const int k2 = k + kWaves;
+                        if (k2 < N / 2)
                         {
-                            const T x = cp[i];
-                            const T y = cq[i];
-                            spp += x * x;
-                            sqq += y * y;
-                            spq += x * y;
+                            const int p = dev::circle_at_fast(k2, r, N);
+                            const int q = dev::circle_at_fast(N - 1 - k2, r, N);
+                            if (p < n && q < n)
+                            {
+                                if (p0 < 0)
+                                {
+                                    p0 = p;
+                                    q0 = q;
+                                }
+                                else
+                                {
+                                    p1 = p;
+                                    q1 = q;
+                                }
+                            }
                         }
-                        spp = dev::wave_sum_dpp(spp);
-                        sqq = dev::wave_sum_dpp(sqq);
-                        spq = dev::wave_sum_dpp(spq);
-                        double c, sn;
-                        if (dev::rotation_params(static_cast<double>(spp), static_cast<double>(sqq),
-                                                 static_cast<double>(spq), eps, c, sn))
+                        if (p1 >= 0)
                         {
-                            const T ct = static_cast<T>(c);
-                            const T st = static_cast<T>(sn);
-                            for (int i = lane; i < m; i += 64)
-                            {
-                                const T x = cp[i];
-                                const T y = cq[i];
-                                cp[i] = ct * x - st * y;
-                                cq[i] = st * x + ct * y;
-                            }
-                            T *vp = v + p * ldv;
-                            T *vq = v + q * ldv;
-                            for (int i = lane; i < n; i += 64)
-                            {
-                                const T x = vp[i];
-                                const T y = vq[i];
-                                vp[i] = ct * x - st * y;
-                                vq[i] = st * x + ct * y;
-                            }
-                            if (lane == 0)
-                            {
-                                atomicAdd(sweep_rot, 1);
-                            }
+                            const int ps[2] = {p0, p1};
+                            const int qs[2] = {q0, q1};
+                            rotate_pairs<T, 2>(a, v, lda, ldv, m, n, ps, qs, eps, lane, sweep_rot);
+                        }
+                        else if (p0 >= 0)
+                        {
+                            const int ps[1] = {p0};
+                            const int qs[1] = {q0};
+                            rotate_pairs<T, 1>(a, v, lda, ldv, m, n, ps, qs, eps, lane, sweep_rot);
                         }
                     }
                     __syncthreads();

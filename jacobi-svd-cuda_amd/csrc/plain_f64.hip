@@ -108,14 +108,15 @@ namespace jsvd
             v[idx] = (row == col) ? 1.0 : 0.0;
         }
 
-        // Block-wide sum of three doubles.  Deterministic: fixed butterfly inside each wavefront, then the
-        // per-wave totals are added in wave order.  Every thread returns the totals.
+        // Block-wide sum of three doubles.  Deterministic: a fixed DPP row-move tree inside each wavefront (no
+        // LDS crossbar round trips on the reduction's dependent chain), then the per-wave totals are added in
+        // wave order.  Every thread returns the totals.
         template <int T>
         __device__ inline void block_sum3(double &x, double &y, double &z, double (*scratch)[3])
         {
-            x = dev::wave_sum(x);
-            y = dev::wave_sum(y);
-            z = dev::wave_sum(z);
+            x = dev::wave_sum_dpp(x);
+            y = dev::wave_sum_dpp(y);
+            z = dev::wave_sum_dpp(z);
             if constexpr (T > 64)
             {
                 constexpr int W = T / 64;
