@@ -299,6 +299,33 @@ namespace jsvd
             n22 = fma(su, h21, cu * h22);
         }
 
+        // v moved one lane along the whole wave by DPP (CTRL 0x138: from lane - 1, 0x130: from lane + 1), or the
+        // lane's own v where `own`.  A DPP move reads its source lane only while that lane is active, so the
+        // move runs on every lane before the select (the empty asm keeps the compiler from sinking it into a
+        // branch of the select).
+        template <int CTRL, typename T>
+        __device__ inline T shifted_or_own(T v, bool own)
+        {
+            T moved = dev::dpp_move<CTRL, 0xF>(v);
+            asm volatile("" : "+v"(moved));
+            return own ? v : moved;
+        }
+
+        // J_t^T G J_u of one 2x2 block, or an exact copy of it when neither pair rotates.  Both rotations are
+        // loaded and applied unconditionally and the copy is a select, so the LDS reads of the block and of the
+        // rotations go out together (one LDS round trip on the inner round's dependent chain, not two).
+        template <typename T>
+        __device__ inline void rotate_or_copy(T g11, T g12, T g21, T g22, const T (&rt)[2], const T (&ru)[2],
+                                              bool rotates, T &n11, T &n12, T &n21, T &n22)
+        {
+            T r11, r12, r21, r22;
+            rotate_block(rt[0], rt[1], ru[0], ru[1], g11, g12, g21, g22, r11, r12, r21, r22);
+            n11 = rotates ? r11 : g11;
+            n12 = rotates ? r12 : g12;
+            n21 = rotates ? r21 : g21;
+            n22 = rotates ? r22 : g22;
+        }
+
         // One inner round on a thread's Q registers: rotate its QP position pairs, then move the positions one
         // step along the circle (values crossing a segment boundary come from the neighbouring lanes).
         template <typename T, int QP, int NSEG>
@@ -546,12 +573,9 @@ namespace jsvd
                             const int o12 = L::at(c1, c4);
                             const int o21 = L::at(c2, c3);
                             const int o22 = L::at(c2, c4);
-                            T n11 = gc[o11], n12 = gc[o12], n21 = gc[o21], n22 = gc[o22];
-                            if (((m >> t) | (m >> u)) & 1ull)
-                            {
-                                rotate_block(cs[b][t][0], cs[b][t][1], cs[b][u][0], cs[b][u][1], n11, n12, n21, n22,
-                                             n11, n12, n21, n22);
-                            }
+                            T n11, n12, n21, n22;
+                            rotate_or_copy<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u],
+                                              ((m >> t) | (m >> u)) & 1ull, n11, n12, n21, n22);
                             gn[o11] = n11;
                             gn[o12] = n12;
                             gn[o22] = n22;
@@ -592,12 +616,9 @@ namespace jsvd
                                 const int o12 = L::at(c1, c4);
                                 const int o21 = L::at(c2, c3);
                                 const int o22 = L::at(c2, c4);
-                                T n11 = gc[o11], n12 = gc[o12], n21 = gc[o21], n22 = gc[o22];
-                                if (((m >> t) | (m >> u)) & 1ull)
-                                {
-                                    rotate_block(cs[b][t][0], cs[b][t][1], cs[b][u][0], cs[b][u][1], n11, n12, n21,
-                                                 n22, n11, n12, n21, n22);
-                                }
+                                T n11, n12, n21, n22;
+                                rotate_or_copy<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u],
+                                                  ((m >> t) | (m >> u)) & 1ull, n11, n12, n21, n22);
                                 gn[o11] = n11;
                                 gn[o12] = n12;
                                 gn[o22] = n22;
@@ -622,30 +643,39 @@ namespace jsvd
                 }
                 if (tid < 64 && rr + 1 < KW - 1)
                 {
-                    // New diagonal entries of this lane's pair (its diagonal block; exact copy when not rotating).
-                    T d11, d12, d21, d22;
-                    rotate_block(my_c, my_s, my_c, my_s, gpp, gpq, gpq, gqq, d11, d12, d21, d22);
                     // The pair of round rr+1: the columns now at positions pu and kw - 1 - pu came from positions
                     // pa and pb, i.e. from pairs ta (first column when pa < kb) and tb of round rr (ta != tb).
                     const int pa = previous_position<KW>(pu), pb = previous_position<KW>(KW - 1 - pu);
                     const int ta = pa < kPairs ? pa : KW - 1 - pa, tb = pb < kPairs ? pb : KW - 1 - pb;
                     const bool fa = pa < kPairs, fb = pb < kPairs;
-                    const T a1 = __shfl(d11, ta, 64), a2 = __shfl(d22, ta, 64);
-                    const T b1 = __shfl(d11, tb, 64), b2 = __shfl(d22, tb, 64);
-                    const int t1 = min(ta, tb), t2 = max(ta, tb);
-                    const T c1 = __shfl(my_c, t1, 64), s1 = __shfl(my_s, t1, 64);
-                    const T c2 = __shfl(my_c, t2, 64), s2 = __shfl(my_s, t2, 64);
-                    // Their new off-diagonal entry: block (t1, t2) of round rr, as the G threads compute it.
-                    const int r1 = dev::circle_at_fast(t1, rr, KW), r2 = dev::circle_at_fast(KW - 1 - t1, rr, KW);
-                    const int k1 = dev::circle_at_fast(t2, rr, KW), k2 = dev::circle_at_fast(KW - 1 - t2, rr, KW);
+                    // ta = pu - 1 and tb = pu + 1, except ta = pu at pu = 0 and tb = pu at pu = kb - 1, so ta < tb
+                    // always.  Their new off-diagonal entry is block (ta, tb) of round rr as the G threads compute
+                    // it; its four G entries are read first, so the LDS reads overlap the lane moves below.
+                    const int r1 = dev::circle_at_fast(ta, rr, KW), r2 = dev::circle_at_fast(KW - 1 - ta, rr, KW);
+                    const int k1 = dev::circle_at_fast(tb, rr, KW), k2 = dev::circle_at_fast(KW - 1 - tb, rr, KW);
+                    const T e11 = gc[L::at(r1, k1)], e12 = gc[L::at(r1, k2)];
+                    const T e21 = gc[L::at(r2, k1)], e22 = gc[L::at(r2, k2)];
+                    // New diagonal entries of this lane's pair (its diagonal block; exact copy when not rotating).
+                    T d11, d12, d21, d22;
+                    rotate_block(my_c, my_s, my_c, my_s, gpp, gpq, gpq, gqq, d11, d12, d21, d22);
+                    // The diagonal entries and rotations of pairs ta and tb come from the neighbouring lanes by
+                    // whole-wave one-lane DPP shifts (wave_shr:1, wave_shl:1), not LDS-crossbar shuffles.  Lanes
+                    // >= kb repeat lanes pu = tid % kb and take the same values.
+                    const bool first = pu == 0, last = pu == kPairs - 1;
+                    const T a1 = shifted_or_own<0x138>(d11, first);
+                    const T a2 = shifted_or_own<0x138>(d22, first);
+                    const T b1 = shifted_or_own<0x130>(d11, last);
+                    const T b2 = shifted_or_own<0x130>(d22, last);
+                    const T c1 = shifted_or_own<0x138>(my_c, first);
+                    const T s1 = shifted_or_own<0x138>(my_s, first);
+                    const T c2 = shifted_or_own<0x130>(my_c, last);
+                    const T s2 = shifted_or_own<0x130>(my_s, last);
                     T f11, f12, f21, f22;
-                    rotate_block(c1, s1, c2, s2, gc[L::at(r1, k1)], gc[L::at(r1, k2)], gc[L::at(r2, k1)],
-                                 gc[L::at(r2, k2)], f11, f12, f21, f22);
-                    const bool row_first = (ta == t1) ? fa : fb; // role of the column from pair t1
-                    const bool col_first = (ta == t1) ? fb : fa; // role of the column from pair t2
+                    rotate_block(c1, s1, c2, s2, e11, e12, e21, e22, f11, f12, f21, f22);
                     gpp = fa ? a1 : a2;
                     gqq = fb ? b1 : b2;
-                    gpq = row_first ? (col_first ? f11 : f12) : (col_first ? f21 : f22);
+                    // the column from pair ta indexes the row of the block, the one from pair tb its column
+                    gpq = fa ? (fb ? f11 : f12) : (fb ? f21 : f22);
                     double c, s;
                     const bool rot = block_rotation<T>(static_cast<double>(gpp), static_cast<double>(gqq),
                                                        static_cast<double>(gpq), eps, c, s);
