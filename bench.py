@@ -196,8 +196,8 @@ def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
         shape = b.shape()
     finally:
         b.close()
-    s_all = d[3].cpu().numpy()
-    sweeps = d[5].cpu().numpy()
+    s_all = d[2].cpu().numpy()
+    sweeps = d[4].cpu().numpy()
     worst64 = worst32 = 0.0
     for q in range(min(checked, count)):
         it = pk.items[q]
