@@ -163,6 +163,29 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
             "rotated_pairs": rotated, "pairs": pairs}
 
 
+def device_copy_bandwidth(torch, mib=512, reps=5):
+    """Achievable HBM bandwidth of a plain device-to-device copy (torch copy_, read + write bytes) on a buffer
+    twice the 256 MiB MALL, timed with events on the current stream: the practical roof for kernels that
+    read and write the same number of bytes, beside the 8 TB/s peak."""
+    src = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    gbs = 2 * src.numel() * 8 / (ms * 1e-3) / 1e9
+    del src, dst
+    return {"bytes_per_copy": 2 * mib * 1024 * 1024, "avg_us": ms * 1e3, "achieved_gbs": gbs,
+            "frac_of_peak": gbs / HBM_PEAK_GBS}
+
+
 def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
     """Batched persistent launch over `count` uneven matrices (m ~ U{8..256}, n ~ U{4..min(m,128)}, entries
     N(0,1) x 10^U(-3,3), alternate items FP32 tol 1e-6 / FP64 tol 1e-14), inputs resident in HBM, queue in cost
@@ -458,6 +481,14 @@ def main() -> None:
     log("[bench] round-kernel roofline on the HBM-resident 16384x4096 shape")
     roofline_hbm = round_kernel_hbm_roofline(torch, jsvd)
     torch.cuda.empty_cache()
+    log("[bench] device-to-device copy bandwidth (512 MiB, HBM-resident)")
+    copy = device_copy_bandwidth(torch)
+    torch.cuda.empty_cache()
+    roofline_hbm["device_copy_gbs"] = copy["achieved_gbs"]
+    roofline_hbm["frac_of_device_copy"] = roofline_hbm["achieved"] / copy["achieved_gbs"]
+    roofline_plain["device_copy_gbs"] = copy["achieved_gbs"]
+    roofline_plain["frac_of_device_copy"] = roofline_plain["achieved"] / copy["achieved_gbs"]
+    log(f"[bench] device copy: {copy}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -495,6 +526,7 @@ def main() -> None:
         "roofline": roofline,
         "roofline_plain": roofline_plain,
         "roofline_plain_hbm_resident": roofline_hbm,
+        "device_copy": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
