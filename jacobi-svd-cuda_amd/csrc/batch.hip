@@ -3,7 +3,8 @@
 // Each workgroup repeatedly takes the next item index from a device counter and solves that matrix alone: the
 // reference algorithm (round-robin pair order, rotation formula and skip test, rotation-free-sweep
 // convergence, sigma/U extraction with the absolute epsilon cutoff) in FP32 or FP64 storage and accumulation,
-// with the rotation parameters always in FP64.  Column pairs of a round are spread over the workgroup's
+// with the rotation parameters always in FP64 (FP64 items: the reference formula; FP32 items: the same skip test
+// and rotation by a shorter chain).  Column pairs of a round are spread over the workgroup's
 // wavefronts (pair k -> wave k % waves), each pair reduced within its wavefront by DPP row moves, so an item's
 // arithmetic does not depend on the workgroup size, the queue order or the other items.
 //
@@ -74,8 +75,18 @@ namespace jsvd
                 sqq[t] = dev::wave_sum_dpp(sqq[t]);
                 spq[t] = dev::wave_sum_dpp(spq[t]);
                 double c = 1.0, sn = 0.0;
-                go[t] = dev::rotation_params(static_cast<double>(spp[t]), static_cast<double>(sqq[t]),
-                                             static_cast<double>(spq[t]), eps, c, sn);
+                if constexpr (sizeof(T) == 4)
+                {
+                    // FP32 items apply their rotations in FP32: the shorter FP64 chain (same skip test, one
+                    // square root and one reciprocal square root) is exact enough there
+                    go[t] = dev::rotation_params_fast(static_cast<double>(spp[t]), static_cast<double>(sqq[t]),
+                                                      static_cast<double>(spq[t]), eps, c, sn);
+                }
+                else
+                {
+                    go[t] = dev::rotation_params(static_cast<double>(spp[t]), static_cast<double>(sqq[t]),
+                                                 static_cast<double>(spq[t]), eps, c, sn);
+                }
                 ct[t] = static_cast<T>(c);
                 st[t] = static_cast<T>(sn);
                 rotated += go[t] ? 1 : 0;
