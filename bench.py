@@ -270,8 +270,10 @@ def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=4):
         "seconds_solve": r["seconds_solve"],
         "seconds_output": r["seconds_output"],
         "load_hidden": r["seconds_load"] <= r["seconds_solve"],
+        "output_hidden": r["seconds_output"] <= r["seconds_solve"],
         "sigma_equals_resident_solve": bool(same),
-        "note": "file in / file out; the load of record k+1 runs on a copy stream while record k is solved",
+        "note": "file in / file out; the load of record k+1 runs on a copy stream and the encode / copy back / "
+                "append of record k-1 on an output stream and a writer thread while record k is solved",
     }
 
 

@@ -348,7 +348,8 @@ typedef struct jsvd_mat_stream_report
     double seconds_total;     /* open to last result written */
     double seconds_load;      /* loader thread: mmap -> pinned copy, H2D copy and device decode */
     double seconds_solve;     /* solver thread: solves (the load of the next record overlaps these) */
-    double seconds_output;    /* solver thread: device encode, copy back, append to the output file */
+    double seconds_output;    /* writer thread: wait for a record's encoded results, append them to the output
+                                 file (overlaps the next record's solve) */
     int32_t sweeps_fp32_last; /* sweeps of the last record */
     int32_t sweeps_fp64_last;
     int64_t total_sweeps;     /* FP32 + FP64 sweeps over all records */
@@ -361,9 +362,11 @@ typedef struct jsvd_mat_stream_report
  * the record reader of src/io.cu:1101-1168 and the packets of src/pipeline.cpp:711-739).  The file is
  * memory-mapped; a loader thread copies record k+1 into a pinned staging buffer (two, alternating), sends it to
  * HBM on a copy stream and decodes it there (big-endian to double) while record k is solved on a compute
- * stream.  config / mixed select the solver (NULL: defaults, plain FP64); records narrower than 33 columns take
- * the plain path.  Every record's shape is validated before the first solve (m >= n > 0, else
- * JSVD_ERR_INVALID_ARGUMENT); truncated files fail with the reader's messages.
+ * stream; record k's U, sigma and V are encoded big-endian and copied back on an output stream and appended by
+ * a writer thread while record k + 1 is solved (two result slots).  config / mixed select the solver (NULL:
+ * defaults, plain FP64); records narrower than 33 columns take the plain path.  Every record's shape is
+ * validated before the first solve (m >= n > 0, else JSVD_ERR_INVALID_ARGUMENT); truncated files fail with the
+ * reader's messages.
  */
 int jsvd_mat_stream_solve(const char *input_path, const char *output_path, const jsvd_config *config,
                           const jsvd_mixed_config *mixed, jsvd_mat_stream_report *report);

@@ -8,12 +8,15 @@
 //   --decode kernel (big-endian u64 -> double), copy stream--> A[k % 2]   ... event ready[k % 2]
 //
 // The solver thread waits for the slot, solves it through a plan on the compute stream (plans are cached per
-// shape), encodes U, sigma (1 x n) and V big-endian on the device, copies them back and appends the three
-// records to the output mapping in input order (the reference pipeline's output packets).
+// shape) into one of two result slots, and hands the slot to an output stream that encodes U, sigma (1 x n)
+// and V big-endian on the device and copies them to pinned memory; a writer thread appends the three records
+// to the output mapping in input order (the reference pipeline's output packets) while the next record is
+// solved.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <condition_variable>
+#include <deque>
 #include <exception>
 #include <map>
 #include <memory>
@@ -189,15 +192,31 @@ namespace jsvd
                     return;
                 }
                 const std::size_t in_bytes = max_count_ * kValueBytes;
-                PinnedBytes staging0(in_bytes), staging1(in_bytes), result(max_out_ * kValueBytes);
+                PinnedBytes staging0(in_bytes), staging1(in_bytes);
                 DeviceBytes raw0(in_bytes), raw1(in_bytes), a0(in_bytes), a1(in_bytes);
-                DeviceBytes d_u(max_count_ * kValueBytes), d_s(max_out_ * kValueBytes), d_v(max_out_ * kValueBytes);
-                DeviceBytes d_enc(max_out_ * kValueBytes);
                 void *staging[2] = {staging0.p, staging1.p};
                 void *raw[2] = {raw0.p, raw1.p};
                 double *a[2] = {static_cast<double *>(a0.p), static_cast<double *>(a1.p)};
-                Stream copy, compute;
+                Stream copy, compute, outs;
                 Event ready[2];
+                // Result slots, two alternating: U / sigma / V of record k in slot k % 2 on the device, encoded
+                // big-endian and copied to pinned memory on the output stream, appended by the writer thread
+                // while record k + 1 is solved.
+                const std::size_t ub = max_count_ * kValueBytes, ob = max_out_ * kValueBytes;
+                DeviceBytes d_u0(ub), d_u1(ub), d_s0(ob), d_s1(ob), d_v0(ob), d_v1(ob), d_enc0(ob), d_enc1(ob);
+                std::unique_ptr<PinnedBytes> result0, result1;
+                if (out)
+                {
+                    result0 = std::make_unique<PinnedBytes>(ob);
+                    result1 = std::make_unique<PinnedBytes>(ob);
+                }
+                double *d_u[2] = {static_cast<double *>(d_u0.p), static_cast<double *>(d_u1.p)};
+                double *d_s[2] = {static_cast<double *>(d_s0.p), static_cast<double *>(d_s1.p)};
+                double *d_v[2] = {static_cast<double *>(d_v0.p), static_cast<double *>(d_v1.p)};
+                unsigned long long *d_enc[2] = {static_cast<unsigned long long *>(d_enc0.p),
+                                                static_cast<unsigned long long *>(d_enc1.p)};
+                void *result[2] = {result0 ? result0->p : nullptr, result1 ? result1->p : nullptr};
+                Event solved[2], copied[2];
 
                 // Slot hand-over between the loader and this thread.
                 std::mutex mu;
@@ -269,16 +288,90 @@ namespace jsvd
                     }
                 } joiner{loader, mu, cv, stop};
 
+                // Writer thread: appends the results of each record in input order once its copy to pinned
+                // memory is done, then frees the slot.
+                std::deque<std::size_t> to_write;
+                bool out_busy[2] = {false, false};
+                std::size_t written = 0;
+                bool writer_stop = false;
+                std::exception_ptr writer_error;
+                double output_seconds = 0.0;
+                std::thread writer([&] {
+                    try
+                    {
+                        for (;;)
+                        {
+                            std::size_t k = 0;
+                            {
+                                std::unique_lock<std::mutex> lock(mu);
+                                cv.wait(lock, [&] { return writer_stop || !to_write.empty(); });
+                                if (to_write.empty())
+                                {
+                                    return; // stopped and drained
+                                }
+                                k = to_write.front();
+                                to_write.pop_front();
+                            }
+                            const int j = static_cast<int>(k & 1);
+                            const auto tw = std::chrono::steady_clock::now();
+                            JSVD_HIP_CHECK(hipEventSynchronize(copied[j].e));
+                            const MatRecord &r = records_[k];
+                            const std::size_t nu = r.count(), ns = r.columns;
+                            const auto *bytes = static_cast<const std::byte *>(result[j]);
+                            append_record(*out, r.rows, r.columns, bytes);
+                            append_record(*out, 1, r.columns, bytes + nu * kValueBytes);
+                            append_record(*out, r.columns, r.columns, bytes + (nu + ns) * kValueBytes);
+                            {
+                                std::lock_guard<std::mutex> lock(mu);
+                                output_seconds += seconds_since(tw);
+                                out_busy[j] = false;
+                                ++written;
+                            }
+                            cv.notify_all();
+                        }
+                    }
+                    catch (...)
+                    {
+                        std::lock_guard<std::mutex> lock(mu);
+                        writer_error = std::current_exception();
+                        cv.notify_all();
+                    }
+                });
+                struct WriterJoiner
+                {
+                    std::thread &t;
+                    std::mutex &mu;
+                    std::condition_variable &cv;
+                    bool &stop;
+                    ~WriterJoiner()
+                    {
+                        {
+                            std::lock_guard<std::mutex> lock(mu);
+                            stop = true;
+                        }
+                        cv.notify_all();
+                        t.join();
+                    }
+                } writer_joiner{writer, mu, cv, writer_stop};
+
                 std::map<std::pair<std::size_t, std::size_t>, PlanPtr> plans;
                 for (std::size_t k = 0; k < records_.size(); ++k)
                 {
                     const int slot = static_cast<int>(k & 1);
                     {
+                        // the input slot holds record k, and the result slot's previous record (k - 2) is written
                         std::unique_lock<std::mutex> lock(mu);
-                        cv.wait(lock, [&] { return loader_error || loaded[slot] == static_cast<long long>(k); });
+                        cv.wait(lock, [&] {
+                            return loader_error || writer_error ||
+                                   (loaded[slot] == static_cast<long long>(k) && !out_busy[slot]);
+                        });
                         if (loader_error)
                         {
                             std::rethrow_exception(loader_error);
+                        }
+                        if (writer_error)
+                        {
+                            std::rethrow_exception(writer_error);
                         }
                     }
                     const MatRecord &r = records_[k];
@@ -298,9 +391,9 @@ namespace jsvd
                         plan.reset(p);
                     }
                     JSVD_HIP_CHECK(hipStreamWaitEvent(compute.s, ready[slot].e, 0));
-                    double *u = static_cast<double *>(d_u.p);
-                    double *s = static_cast<double *>(d_s.p);
-                    double *v = static_cast<double *>(d_v.p);
+                    double *u = d_u[slot];
+                    double *s = d_s[slot];
+                    double *v = d_v[slot];
                     int32_t sweeps = 0;
                     check_status(jsvd_plan_solve_device(plan.get(), a[slot], u, s, v, &sweeps));
                     jsvd_solve_info info{};
@@ -312,27 +405,30 @@ namespace jsvd
                     }
                     cv.notify_all();
 
-                    const auto to = std::chrono::steady_clock::now();
                     if (out)
                     {
+                        // encode and copy back on the output stream after the solve; the writer appends
                         const std::size_t nu = r.count(), ns = r.columns, nv = r.columns * r.columns;
-                        auto *enc = static_cast<unsigned long long *>(d_enc.p);
-                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(nu)), dim3(256), 0, compute.s, u, enc,
+                        unsigned long long *enc = d_enc[slot];
+                        JSVD_HIP_CHECK(hipEventRecord(solved[slot].e, compute.s));
+                        JSVD_HIP_CHECK(hipStreamWaitEvent(outs.s, solved[slot].e, 0));
+                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(nu)), dim3(256), 0, outs.s, u, enc,
                                            static_cast<int64_t>(nu));
-                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(ns)), dim3(256), 0, compute.s, s,
+                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(ns)), dim3(256), 0, outs.s, s,
                                            enc + nu, static_cast<int64_t>(ns));
-                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(nv)), dim3(256), 0, compute.s, v,
+                        hipLaunchKernelGGL(encode_be_kernel, dim3(grid_for(nv)), dim3(256), 0, outs.s, v,
                                            enc + nu + ns, static_cast<int64_t>(nv));
                         JSVD_HIP_CHECK(hipGetLastError());
-                        JSVD_HIP_CHECK(hipMemcpyAsync(result.p, enc, (nu + ns + nv) * kValueBytes,
-                                                      hipMemcpyDeviceToHost, compute.s));
-                        JSVD_HIP_CHECK(hipStreamSynchronize(compute.s));
-                        const auto *bytes = static_cast<const std::byte *>(result.p);
-                        append_record(*out, r.rows, r.columns, bytes);
-                        append_record(*out, 1, r.columns, bytes + nu * kValueBytes);
-                        append_record(*out, r.columns, r.columns, bytes + (nu + ns) * kValueBytes);
+                        JSVD_HIP_CHECK(hipMemcpyAsync(result[slot], enc, (nu + ns + nv) * kValueBytes,
+                                                      hipMemcpyDeviceToHost, outs.s));
+                        JSVD_HIP_CHECK(hipEventRecord(copied[slot].e, outs.s));
+                        {
+                            std::lock_guard<std::mutex> lock(mu);
+                            out_busy[slot] = true;
+                            to_write.push_back(k);
+                        }
+                        cv.notify_all();
                     }
-                    rep.seconds_output += seconds_since(to);
                     rep.records += 1;
                     rep.payload_bytes += static_cast<int64_t>(r.count() * kValueBytes);
                     rep.sweeps_fp32_last = info.sweeps_fp32;
@@ -341,11 +437,21 @@ namespace jsvd
                 }
                 if (out)
                 {
+                    std::unique_lock<std::mutex> lock(mu);
+                    cv.wait(lock, [&] { return writer_error || written == records_.size(); });
+                    if (writer_error)
+                    {
+                        std::rethrow_exception(writer_error);
+                    }
+                }
+                if (out)
+                {
                     out->flush();
                 }
                 {
                     std::lock_guard<std::mutex> lock(mu);
                     rep.seconds_load = load_seconds;
+                    rep.seconds_output = output_seconds;
                 }
                 rep.seconds_total = seconds_since(t0);
             }
