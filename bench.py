@@ -77,6 +77,7 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
     t1.record(stream)
     torch.cuda.synchronize()
     avg_us = 1000.0 * t0.elapsed_time(t1) / launches
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
     flops = pairs * EIGEN_FLOPS_PER_PAIR
     achieved = flops / (avg_us * 1e-6) / 1e12
     return {
@@ -92,8 +93,12 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
         "algorithmic_flops_per_launch": flops,
         "pairs_per_launch": pairs,
         "rotated_pairs_per_block_pair": int(rot.float().mean().item()),
-        "note": "issue-bound on vector instructions (index arithmetic and 2x2 updates), far from either roof; "
-                "counted against the FP32 matrix peak",
+        "occupied_cus": pairs,
+        "frac_of_occupied_cus": achieved / (FP32_PEAK_TFLOPS * pairs / cus),
+        "note": "one 1024-thread workgroup per block pair, so one launch occupies `pairs` of the chip's CUs; a "
+                "dependent chain of 63 inner rounds bound by LDS instruction throughput and latency, far from "
+                "either roof; counted against the FP32 matrix peak of the whole chip (frac) and of the CUs it "
+                "occupies (frac_of_occupied_cus)",
     }
 
 

@@ -226,7 +226,9 @@ typedef struct jsvd_batch_buffers
 
 typedef struct jsvd_batch jsvd_batch;
 
-/* Batch solver for matrices up to max_rows x max_columns (max_columns <= max_rows <= 4096). */
+/* Batch solver for matrices up to max_rows x max_columns (max_columns <= max_rows <= 4096).  Items whose A or
+ * V do not fit the workgroup's LDS use a global scratch slice per resident workgroup; that area is capped at
+ * 8 GiB per batch object (very large limits run fewer resident workgroups). */
 int jsvd_batch_create(int64_t max_rows, int64_t max_columns, void *stream, jsvd_batch **batch);
 int jsvd_batch_destroy(jsvd_batch *batch);
 

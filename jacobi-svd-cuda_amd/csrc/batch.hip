@@ -343,6 +343,7 @@ namespace jsvd
             BatchKernel kernel;
         };
         constexpr int kKiB = 1024;
+        constexpr int64_t kScratchBudgetBytes = int64_t(8) << 30; // global scratch of one batch object
         const BatchShape kShapes[] = {
             {256, 76 * kKiB, batch_kernel<256, 76 * kKiB>},     // default: 2 workgroups x 4 waves per CU
             {128, 38 * kKiB, batch_kernel<128, 38 * kKiB>},     // 4 x 2 waves, more items in flight
@@ -407,7 +408,10 @@ struct jsvd_batch
         int per_cu = 0;
         JSVD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(sh.kernel), sh.threads, 0));
-        const int g = std::max(per_cu, 1) * cus;
+        // One resident wave of workgroups, fewer where their scratch slices would exceed the budget (large
+        // limits: a 4096 x 4096 slice is 268 MB); the persistent workgroups drain the queue either way.
+        const int64_t by_budget = std::max<int64_t>(1, jsvd::kScratchBudgetBytes / (sizeof(double) * scratch_per_wg));
+        const int g = static_cast<int>(std::min<int64_t>(std::max(per_cu, 1) * cus, by_budget));
         if (g > scratch_wgs)
         {
             JSVD_HIP_CHECK(hipStreamSynchronize(stream)); // launches in flight may still use the old area
