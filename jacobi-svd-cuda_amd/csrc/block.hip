@@ -217,17 +217,19 @@ namespace jsvd
         };
 
         // LDS image of the symmetric kw x kw G, one slot per unordered entry.  The upper triangle of a square
-        // array with a padded row, except for FP64 at kb = 64, where two buffers of that would not fit: entry
-        // {i, j} at row i, offset d = (j - i) mod kw for d < kw / 2 (row min(i, j) for d = kw / 2), else at row
-        // j, offset kw - d, in rows of kw / 2 + 2 slots.  The even row length keeps both address steps that
-        // neighbouring lanes take (+1 along a row, row length - 1 across rows) odd, so a wave's accesses spread
-        // over all LDS banks (an odd row length makes the second step a multiple of the bank count).
+        // array with padded rows (kw + 2 slots at kb <= 32, kw + 1 at kb = 64; measured: kw + 2 takes 5 % off an
+        // inner round at kb = 32 in FP32 and FP64 and costs 4 % at kb = 64), except for FP64 at kb = 64, where two
+        // buffers of that would not fit: entry {i, j} at row i, offset d = (j - i) mod kw for d < kw / 2 (row
+        // min(i, j) for d = kw / 2), else at row j, offset kw - d, in rows of kw / 2 + 2 slots.  There the even
+        // row length keeps both address steps that neighbouring lanes take (+1 along a row, row length - 1 across
+        // rows) odd, so a wave's accesses spread over all LDS banks (an odd row length makes the second step a
+        // multiple of the bank count).
         template <typename T, int KB>
         struct GramLds
         {
             static constexpr int KW = 2 * KB;
             static constexpr bool kCirc = KB == 64 && sizeof(T) == 8;
-            static constexpr int kStride = kCirc ? KW / 2 + 2 : KW + 1;
+            static constexpr int kStride = kCirc ? KW / 2 + 2 : (KB == 64 ? KW + 1 : KW + 2);
             static constexpr int kElems = KW * kStride;
             __device__ static int at(int i, int j)
             {
@@ -299,6 +301,17 @@ namespace jsvd
             n22 = fma(su, h21, cu * h22);
         }
 
+        // v moved one lane along the whole wave by DPP (CTRL 0x138: from lane - 1, 0x130: from lane + 1), on every
+        // lane of the wave: the empty asm keeps the compiler from sinking the move into a branch of a later select
+        // (a DPP move reads nothing from an inactive lane).
+        template <int CTRL, typename T>
+        __device__ inline T full_wave_shift(T v)
+        {
+            T moved = dev::dpp_move<CTRL, 0xF>(v);
+            asm volatile("" : "+v"(moved));
+            return moved;
+        }
+
         // v moved one lane along the whole wave by DPP (CTRL 0x138: from lane - 1, 0x130: from lane + 1), or the
         // lane's own v where `own`.  A DPP move reads its source lane only while that lane is active, so the
         // move runs on every lane before the select (the empty asm keeps the compiler from sinking it into a
@@ -343,8 +356,10 @@ namespace jsvd
             const T out_lo = lo[QP - 1];
             // position kw - 1 - QP s -> kw - QP s (previous segment; segment 0: position 1, its own lo[1])
             const T out_hi = hi[0];
-            const T from_prev = __shfl_up(out_lo, 1, 64);
-            const T from_next = __shfl_down(out_hi, 1, 64);
+            // lane - 1 and lane + 1 by whole-wave DPP shifts (the Q waves run this with every lane active; the
+            // lanes whose shifted value would come from outside the wave do not use it)
+            const T from_prev = full_wave_shift<0x138>(out_lo);
+            const T from_next = full_wave_shift<0x130>(out_hi);
             const T lo0 = lo[0];
 #pragma unroll
             for (int i = QP - 1; i >= 1; --i)
