@@ -893,10 +893,11 @@ namespace jsvd
                 T *x = in_a ? a : v;
                 const int64_t ld = in_a ? lda : ldv;
                 const int64_t r0 = static_cast<int64_t>(in_a ? s : s - slabs_a) * H;
+                const int sn = s + static_cast<int>(gridDim.y);
+                if (sn < total)
                 {
-                    // next slab of this workgroup (the last iteration re-reads the current slab, unused; this keeps
-                    // the prefetch unconditional so `pre` stays in registers)
-                    const int sn = (s + static_cast<int>(gridDim.y) < total) ? s + static_cast<int>(gridDim.y) : s;
+                    // next slab of this workgroup, in flight during this slab's products (a workgroup with one
+                    // slab issues no second load that its end would have to wait for)
                     const bool n_in_a = sn < slabs_a;
                     const T *xn = n_in_a ? a : v;
                     const int64_t ldn = n_in_a ? lda : ldv;
