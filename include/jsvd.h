@@ -58,6 +58,11 @@ typedef struct jsvd_config
 /* Fills *cfg with the reference defaults (include/kernels.hpp:83-123). */
 void jsvd_config_default(jsvd_config *cfg);
 
+/* Checks *cfg as every solve does (the reference's validation, src/kernels.cu:807-833: epsilon > 0,
+ * max_sweeps > 0, layout and benchmark knobs), without a matrix: JSVD_OK or JSVD_ERR_INVALID_ARGUMENT with the
+ * reference's message.  For callers that route work before solving (the pipeline's batched cases). */
+int jsvd_config_validate(const jsvd_config *cfg);
+
 /* Thread-local message of the last failing call on the calling thread ("" if none). */
 const char *jsvd_last_error(void);
 
@@ -247,6 +252,16 @@ int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, i
  */
 int jsvd_batch_solve_async(jsvd_batch *batch, const jsvd_batch_item *d_items, int64_t count,
                            const jsvd_batch_buffers *buf, void *done);
+
+/*
+ * Host-pointer form of jsvd_batch_solve_device: item descriptors and the packed buffers (lengths in elements,
+ * as in jsvd_batch_buffers) live in host memory; the batch object copies them through device staging it keeps
+ * and grows, solves in one persistent launch and copies U, sigma, V and the sweep counts back.  Synchronous.
+ * No reference counterpart; the pipeline's batched cases use it.
+ */
+int jsvd_batch_solve_host(jsvd_batch *batch, const jsvd_batch_item *items, int64_t count, const double *a,
+                          int64_t a_len, double *u, int64_t u_len, double *sigma, int64_t sigma_len, double *v,
+                          int64_t v_len, int32_t *sweeps);
 
 /*
  * Launch shape of the persistent batched kernel (no reference counterpart): threads per workgroup (the

@@ -30,6 +30,13 @@ namespace jacobi::svd::pipeline
         MatrixFileFormat output_format = MatrixFileFormat::auto_detect;
         std::size_t max_queued_results = 4;
         JacobiSvdConfig kernel_config{};
+        // Appended (no reference counterpart): when > 0, runs of consecutive small cases (rows <= 256,
+        // columns <= 128, rows >= columns) are grouped, up to this many, into one persistent batched launch
+        // (jsvd_batch_solve_host) instead of one solver-lane solve each.  Both follow the reference algorithm
+        // (pair order, rotation and skip test, convergence rule), so results agree to rounding with the same
+        // column order and signs; 0 (default) keeps every case on a solver lane, bit-equal to
+        // one_sided_jacobi_svd.
+        std::size_t device_batch_cases = 0;
     };
 
     // Run summary (reference pipeline.hpp:70-111).

@@ -53,6 +53,7 @@ namespace
         JacobiSvdConfig kernel{};
         std::size_t queue_capacity = 4;
         Solver solver = Solver::plain;
+        std::size_t device_batch = 0; // 0: every case on a solver lane
         bool force = false;
         bool dry_run = false;
         bool print_config = false;
@@ -91,6 +92,7 @@ namespace
             {"layout-transpose-bench-sweeps", '\0', "N", "Auto-tune sweep cap per benchmark run (>0)."},
             {"queue-capacity", 'c', "N", "In-flight/reorder window size (>0)."},
             {"solver", '\0', "PATH", "Solver: plain (reference algorithm, default) | block | mixed (*.mat only)."},
+            {"device-batch", '\0', "N", "Plain solver: group up to N consecutive small cases (m<=256, n<=128) per batched launch (>0)."},
             {"force", 'y', "", "Overwrite existing output file."},
             {"dry-run", '\0', "", "Validate arguments and print config only."},
             {"print-config", '\0', "", "Print effective config before execution."},
@@ -240,6 +242,7 @@ namespace
         else if (n == "layout-transpose-bench-sweeps") o.kernel.layout_transpose_benchmark_sweeps = positive_int(v, n);
         else if (n == "queue-capacity") o.queue_capacity = positive_size(v, n);
         else if (n == "solver") o.solver = parse_solver(v, n);
+        else if (n == "device-batch") o.device_batch = positive_size(v, n);
         else if (n == "force") o.force = true;
         else if (n == "dry-run") o.dry_run = true;
         else if (n == "print-config") o.print_config = true;
@@ -364,6 +367,10 @@ namespace
             if (in != MatrixFileFormat::mat || out != MatrixFileFormat::mat)
             {
                 throw UsageError("--solver block|mixed reads and writes *.mat files only.");
+            }
+            if (o.device_batch > 0)
+            {
+                throw UsageError("--device-batch applies to the plain solver only.");
             }
         }
     }
@@ -517,6 +524,7 @@ namespace
                   << "layout-bench-swp: " << k.layout_transpose_benchmark_sweeps << '\n'
                   << "queue-capacity  : " << o.queue_capacity << '\n'
                   << "solver          : " << solver_name(o.solver) << '\n'
+                  << "device-batch    : " << o.device_batch << '\n'
                   << "force-overwrite : " << (o.force ? "true" : "false") << '\n';
     }
 
@@ -622,6 +630,7 @@ int main(int argc, char **argv)
             pc.output_format = o.output_format;
             pc.max_queued_results = o.queue_capacity;
             pc.kernel_config = o.kernel;
+            pc.device_batch_cases = o.device_batch;
             report = jacobi::svd::pipeline::run_pipeline(pc);
         }
         else

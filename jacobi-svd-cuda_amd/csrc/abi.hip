@@ -417,6 +417,17 @@ namespace
 
 extern "C" {
 
+int jsvd_config_validate(const jsvd_config *cfg)
+{
+    return jsvd::guarded([&] {
+        if (!cfg)
+        {
+            jsvd::throw_invalid("config must not be null.");
+        }
+        jsvd::validate_config(*cfg);
+    });
+}
+
 void jsvd_config_default(jsvd_config *cfg)
 {
     if (!cfg)

@@ -386,12 +386,19 @@ struct jsvd_batch
     bool own_stream = false;
     double *scratch = nullptr;
     unsigned long long *counter = nullptr;
+    // device staging of the host-pointer entry point, grown on demand (bytes held in *_cap)
+    void *st_items = nullptr, *st_a = nullptr, *st_u = nullptr, *st_s = nullptr, *st_v = nullptr, *st_sw = nullptr;
+    size_t cap_items = 0, cap_a = 0, cap_u = 0, cap_s = 0, cap_v = 0, cap_sw = 0;
 
     ~jsvd_batch()
     {
         if (stream)
         {
             (void)hipStreamSynchronize(stream);
+        }
+        for (void *p : {st_items, st_a, st_u, st_s, st_v, st_sw})
+        {
+            (void)hipFree(p);
         }
         (void)hipFree(scratch);
         (void)hipFree(counter);
@@ -423,6 +430,21 @@ struct jsvd_batch
         }
         shape = index;
         grid = g;
+    }
+
+    // Grows a staging area to hold `bytes` (the stream is idle: callers synchronise before returning).
+    void ensure(void *&p, size_t &cap, size_t bytes)
+    {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes <= cap)
+        {
+            return;
+        }
+        JSVD_HIP_CHECK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        JSVD_HIP_CHECK(hipMalloc(&p, bytes));
+        cap = bytes;
     }
 
     void enqueue(const jsvd_batch_item *d_items, int64_t count, const jsvd_batch_buffers &buf)
@@ -634,6 +656,53 @@ int jsvd_batch_autotune(jsvd_batch *batch, const jsvd_batch_item *d_items, int64
         {
             *report = rep;
         }
+    });
+}
+
+int jsvd_batch_solve_host(jsvd_batch *batch, const jsvd_batch_item *items, int64_t count, const double *a,
+                          int64_t a_len, double *u, int64_t u_len, double *sigma, int64_t sigma_len, double *v,
+                          int64_t v_len, int32_t *sweeps)
+{
+    return jsvd::guarded([&] {
+        if (!batch)
+        {
+            jsvd::throw_invalid("batch must not be null.");
+        }
+        if (count < 0 || a_len < 0 || u_len < 0 || sigma_len < 0 || v_len < 0)
+        {
+            jsvd::throw_invalid("count and buffer lengths must be non-negative.");
+        }
+        if (count == 0)
+        {
+            return;
+        }
+        if (!items || !a || !u || !sigma || !v || !sweeps)
+        {
+            jsvd::throw_invalid("batch host buffers must not be null.");
+        }
+        const size_t bi = sizeof(jsvd_batch_item) * static_cast<size_t>(count);
+        const size_t ba = sizeof(double) * static_cast<size_t>(a_len), bu = sizeof(double) * static_cast<size_t>(u_len);
+        const size_t bs = sizeof(double) * static_cast<size_t>(sigma_len), bv = sizeof(double) * static_cast<size_t>(v_len);
+        const size_t bw = sizeof(int32_t) * static_cast<size_t>(count);
+        JSVD_HIP_CHECK(hipStreamSynchronize(batch->stream)); // staging may still be read by an earlier call
+        batch->ensure(batch->st_items, batch->cap_items, bi);
+        batch->ensure(batch->st_a, batch->cap_a, ba);
+        batch->ensure(batch->st_u, batch->cap_u, bu);
+        batch->ensure(batch->st_s, batch->cap_s, bs);
+        batch->ensure(batch->st_v, batch->cap_v, bv);
+        batch->ensure(batch->st_sw, batch->cap_sw, bw);
+        hipStream_t s = batch->stream;
+        JSVD_HIP_CHECK(hipMemcpyAsync(batch->st_items, items, bi, hipMemcpyHostToDevice, s));
+        JSVD_HIP_CHECK(hipMemcpyAsync(batch->st_a, a, ba, hipMemcpyHostToDevice, s));
+        const jsvd_batch_buffers buf{static_cast<const double *>(batch->st_a), static_cast<double *>(batch->st_u),
+                                     static_cast<double *>(batch->st_s), static_cast<double *>(batch->st_v),
+                                     static_cast<int32_t *>(batch->st_sw), a_len, u_len, sigma_len, v_len};
+        batch->enqueue(static_cast<const jsvd_batch_item *>(batch->st_items), count, buf);
+        JSVD_HIP_CHECK(hipMemcpyAsync(u, batch->st_u, bu, hipMemcpyDeviceToHost, s));
+        JSVD_HIP_CHECK(hipMemcpyAsync(sigma, batch->st_s, bs, hipMemcpyDeviceToHost, s));
+        JSVD_HIP_CHECK(hipMemcpyAsync(v, batch->st_v, bv, hipMemcpyDeviceToHost, s));
+        JSVD_HIP_CHECK(hipMemcpyAsync(sweeps, batch->st_sw, bw, hipMemcpyDeviceToHost, s));
+        JSVD_HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

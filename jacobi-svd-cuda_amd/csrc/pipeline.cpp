@@ -4,7 +4,9 @@
 // them, and one writer thread emits U, Sigma(1 x n), V per case strictly in input order.  Each lane owns
 // a HIP stream and a solver plan reused while the shape repeats, so several cases run on the GPU at once
 // (the reference's workers all share the legacy default stream and serialise).  The FIFO of pending
-// results holds at most max_queued_results entries; the reader blocks when it is full.
+// results holds at most max_queued_results entries; the reader blocks when it is full.  With
+// device_batch_cases > 0, runs of consecutive small cases go to a lane as one group, solved by one persistent
+// batched launch, each case keeping its own place in the output order.
 #include "pipeline.hpp"
 
 #include <algorithm>
@@ -114,6 +116,10 @@ namespace jacobi::svd::pipeline
                 {
                     (void)jsvd_plan_destroy(plan_);
                 }
+                if (batch_)
+                {
+                    (void)jsvd_batch_destroy(batch_);
+                }
             }
             SolverLane(const SolverLane &) = delete;
             SolverLane &operator=(const SolverLane &) = delete;
@@ -159,12 +165,93 @@ namespace jacobi::svd::pipeline
                 return p;
             }
 
+            // Small cases (batchable(), validated by the caller) in one persistent batched launch; one packet per
+            // case, in the given order.
+            std::vector<Packet> solve_batch(const std::vector<std::shared_ptr<const io::Matrix>> &cases,
+                                            const std::vector<std::size_t> &indices)
+            {
+                if (!batch_)
+                {
+                    const int st = jsvd_batch_create(kBatchRows, kBatchColumns, nullptr, &batch_);
+                    if (st != JSVD_OK)
+                    {
+                        batch_ = nullptr;
+                        raise(st);
+                    }
+                }
+                const std::size_t k = cases.size();
+                std::vector<jsvd_batch_item> items(k);
+                int64_t na = 0, nv = 0, ns = 0;
+                for (std::size_t i = 0; i < k; ++i)
+                {
+                    const io::Matrix &a = *cases[i];
+                    jsvd_batch_item &it = items[i];
+                    it.a_offset = na;
+                    it.u_offset = na;
+                    it.sigma_offset = ns;
+                    it.v_offset = nv;
+                    it.rows = static_cast<int32_t>(a.rows);
+                    it.columns = static_cast<int32_t>(a.columns);
+                    it.precision = JSVD_PRECISION_F64;
+                    it.max_sweeps = cfg_.max_sweeps;
+                    it.tolerance = cfg_.epsilon;
+                    na += static_cast<int64_t>(a.rows * a.columns);
+                    ns += static_cast<int64_t>(a.columns);
+                    nv += static_cast<int64_t>(a.columns * a.columns);
+                }
+                std::vector<double> ha(static_cast<std::size_t>(na)), hu(ha.size()), hs(static_cast<std::size_t>(ns)),
+                    hv(static_cast<std::size_t>(nv));
+                for (std::size_t i = 0; i < k; ++i)
+                {
+                    std::copy(cases[i]->values.begin(), cases[i]->values.end(), ha.begin() + items[i].a_offset);
+                }
+                std::vector<int32_t> sweeps(k);
+                const int st = jsvd_batch_solve_host(batch_, items.data(), static_cast<int64_t>(k), ha.data(), na,
+                                                     hu.data(), na, hs.data(), ns, hv.data(), nv, sweeps.data());
+                if (st != JSVD_OK)
+                {
+                    raise(st);
+                }
+                std::vector<Packet> out(k);
+                for (std::size_t i = 0; i < k; ++i)
+                {
+                    const std::size_t m = cases[i]->rows, n = cases[i]->columns;
+                    if (sweeps[i] < 0)
+                    {
+                        throw std::runtime_error("Testcase[" + std::to_string(indices[i]) +
+                                                 "] was rejected by the batched solver.");
+                    }
+                    Packet &p = out[i];
+                    p.index = indices[i];
+                    p.sweeps = sweeps[i];
+                    const auto u0 = hu.begin() + items[i].u_offset;
+                    const auto s0 = hs.begin() + items[i].sigma_offset;
+                    const auto v0 = hv.begin() + items[i].v_offset;
+                    p.u = io::Matrix{m, n, std::vector<double>(u0, u0 + static_cast<std::ptrdiff_t>(m * n))};
+                    p.sigma = io::Matrix{1, n, std::vector<double>(s0, s0 + static_cast<std::ptrdiff_t>(n))};
+                    p.v = io::Matrix{n, n, std::vector<double>(v0, v0 + static_cast<std::ptrdiff_t>(n * n))};
+                }
+                return out;
+            }
+
+            static constexpr int64_t kBatchRows = 256;
+            static constexpr int64_t kBatchColumns = 128;
+
         private:
             jsvd_config cfg_;
             jsvd_plan *plan_ = nullptr;
+            jsvd_batch *batch_ = nullptr;
             int64_t m_ = 0;
             int64_t n_ = 0;
         };
+
+        // Cases the batched launch takes: well-formed, rows >= columns, within the batch limits.
+        bool batchable(std::size_t rows, std::size_t columns)
+        {
+            return rows >= 1 && columns >= 1 && rows >= columns &&
+                   rows <= static_cast<std::size_t>(SolverLane::kBatchRows) &&
+                   columns <= static_cast<std::size_t>(SolverLane::kBatchColumns);
+        }
 
         // Fixed pool of solver lanes; each worker thread owns one lane for its whole life.
         class LanePool
@@ -203,6 +290,16 @@ namespace jacobi::svd::pipeline
                 }
                 cv_.notify_one();
                 return fut;
+            }
+
+            // A job that reports through its own promises (a batched group of cases).
+            void submit_job(Job job)
+            {
+                {
+                    std::lock_guard<std::mutex> lock(mu_);
+                    jobs_.emplace_back(std::move(job));
+                }
+                cv_.notify_one();
             }
 
         private:
@@ -425,6 +522,70 @@ namespace jacobi::svd::pipeline
         try
         {
             LanePool pool(lane_count(), kernel);
+            // Groups of consecutive small cases for one batched launch each (device_batch_cases > 0 and a valid
+            // kernel config; with an invalid one every case goes to a lane, which reports the error in order).
+            const jsvd_config ckernel = c_config(kernel);
+            const std::size_t group_max =
+                (config_.device_batch_cases > 0 && jsvd_config_validate(&ckernel) == JSVD_OK)
+                    ? config_.device_batch_cases
+                    : 0;
+            std::vector<std::shared_ptr<const io::Matrix>> group;
+            std::vector<std::size_t> group_idx;
+            // The group's job goes to the pool before its results are queued on the writer, so a full writer
+            // queue always waits on work that is already running.
+            auto flush = [&] {
+                if (group.empty())
+                {
+                    return;
+                }
+                auto promises = std::make_shared<std::vector<std::promise<Packet>>>(group.size());
+                std::vector<std::future<Packet>> futures;
+                for (auto &pr : *promises)
+                {
+                    futures.push_back(pr.get_future());
+                }
+                auto cases = std::make_shared<std::vector<std::shared_ptr<const io::Matrix>>>(std::move(group));
+                auto idx = std::make_shared<std::vector<std::size_t>>(std::move(group_idx));
+                group = {};
+                group_idx = {};
+                pool.submit_job([cases, idx, promises](SolverLane &lane) {
+                    try
+                    {
+                        std::vector<Packet> out = lane.solve_batch(*cases, *idx);
+                        for (std::size_t i = 0; i < out.size(); ++i)
+                        {
+                            (*promises)[i].set_value(std::move(out[i]));
+                        }
+                    }
+                    catch (...)
+                    {
+                        const std::exception_ptr e = std::current_exception();
+                        for (auto &pr : *promises)
+                        {
+                            try
+                            {
+                                pr.set_exception(e);
+                            }
+                            catch (...)
+                            {
+                                // already satisfied
+                            }
+                        }
+                    }
+                });
+                for (auto &f : futures)
+                {
+                    writer.submit(std::move(f));
+                }
+            };
+            auto add_to_group = [&](std::shared_ptr<const io::Matrix> a, std::size_t index) {
+                group.push_back(std::move(a));
+                group_idx.push_back(index);
+                if (group.size() >= group_max)
+                {
+                    flush();
+                }
+            };
             if (in_format == MatrixFileFormat::mat)
             {
                 io::MatDispatchReader reader(config_.input_path);
@@ -434,6 +595,25 @@ namespace jacobi::svd::pipeline
                     auto shared = std::make_shared<io::MatDispatchTask>(std::move(task));
                     task = io::MatDispatchTask{};
                     const std::size_t index = shared->sequence_index;
+                    if (group_max > 0 && batchable(shared->rows, shared->columns))
+                    {
+                        std::shared_ptr<const io::Matrix> a;
+                        try
+                        {
+                            a = std::make_shared<const io::Matrix>(io::decode_dispatch_task_matrix(*shared));
+                        }
+                        catch (...)
+                        {
+                            a = nullptr; // the lane path decodes it again and reports the error in order
+                        }
+                        if (a && a->values.size() == a->rows * a->columns)
+                        {
+                            add_to_group(std::move(a), index);
+                            ++submitted;
+                            continue;
+                        }
+                    }
+                    flush();
                     writer.submit(pool.submit([shared, index](SolverLane &lane) {
                         const io::Matrix a = io::decode_dispatch_task_matrix(*shared);
                         return lane.solve(a, index);
@@ -450,10 +630,19 @@ namespace jacobi::svd::pipeline
                     auto shared = std::make_shared<io::Matrix>(std::move(a));
                     a = io::Matrix{};
                     const std::size_t index = submitted;
+                    if (group_max > 0 && batchable(shared->rows, shared->columns) &&
+                        shared->values.size() == shared->rows * shared->columns)
+                    {
+                        add_to_group(std::move(shared), index);
+                        ++submitted;
+                        continue;
+                    }
+                    flush();
                     writer.submit(pool.submit([shared, index](SolverLane &lane) { return lane.solve(*shared, index); }));
                     ++submitted;
                 }
             }
+            flush();
             writer.close_success(submitted);
         }
         catch (...)

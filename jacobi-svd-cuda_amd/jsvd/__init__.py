@@ -167,6 +167,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB_PATH)
         L.jsvd_config_default.argtypes = [ctypes.POINTER(JsvdConfig)]
         L.jsvd_config_default.restype = None
+        L.jsvd_config_validate.argtypes = [ctypes.POINTER(JsvdConfig)]
         L.jsvd_last_error.restype = ctypes.c_char_p
         L.jsvd_version.restype = ctypes.c_char_p
         L.jsvd_plain_f64.argtypes = [_vp, _i64, _i64, ctypes.POINTER(JsvdConfig), _vp, _vp, _vp, _i32p]
@@ -228,6 +229,11 @@ def _as_c_config(config) -> JsvdConfig:
     if isinstance(config, JsvdConfig):
         return config
     return config.to_c()
+
+
+def validate_config(config) -> None:
+    """jsvd_config_validate: raises ValueError with the reference's message for an invalid configuration."""
+    check(lib().jsvd_config_validate(ctypes.byref(_as_c_config(config))))
 
 
 def one_sided_jacobi_svd(a, config: Optional[JacobiSvdConfig] = None) -> JacobiSvdResult:
@@ -478,6 +484,7 @@ def _bind_batch(L):
                                        ctypes.POINTER(ctypes.c_int32)]
     L.jsvd_batch_autotune.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers), ctypes.c_int32,
                                       ctypes.POINTER(BatchTuneReport)]
+    L.jsvd_batch_solve_host.argtypes = [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]
     L._batch_bound = True
 
 
@@ -564,6 +571,21 @@ class Batch:
         buf = BatchBuffers(d_a.data_ptr(), d_u.data_ptr(), d_s.data_ptr(), d_v.data_ptr(), d_sweeps.data_ptr(),
                            d_a.numel(), d_u.numel(), d_s.numel(), d_v.numel())
         check(lib().jsvd_batch_solve_device(self._h, d_items.data_ptr(), int(count), ctypes.byref(buf)))
+
+    def solve_host_packed(self, pk: "BatchPacked", order=None):
+        """Host-pointer solve (jsvd_batch_solve_host) of a packed batch; items in `order` (default: as packed).
+        Returns (u, sigma, v, sweeps) as flat host arrays (sweeps in queue order)."""
+        items = pk.items if order is None else pk.items[order]
+        items = np.ascontiguousarray(items)
+        a = np.ascontiguousarray(pk.a, dtype=np.float64)
+        u = np.zeros(max(pk.lens[1], 1))
+        s = np.zeros(max(pk.lens[2], 1))
+        v = np.zeros(max(pk.lens[3], 1))
+        sw = np.zeros(len(items), dtype=np.int32)
+        check(lib().jsvd_batch_solve_host(self._h, items.ctypes.data, len(items), a.ctypes.data, a.size,
+                                          u.ctypes.data, u.size, s.ctypes.data, s.size, v.ctypes.data, v.size,
+                                          sw.ctypes.data))
+        return u, s, v, sw
 
     def set_shape(self, threads: int = 0, lds_bytes: int = 0):
         """Launch shape (threads per workgroup, LDS bytes per workgroup), one of BATCH_SHAPES; (0, 0) = default."""

@@ -315,6 +315,66 @@ namespace
         }
         check(ordered, "pipeline mat->mat emits U, Sigma(1xn), V per case in input order, equal to direct solves");
 
+        // Batched small cases (device_batch_cases): same order, agreement with direct solves to rounding (the
+        // same reference algorithm through the persistent batched kernel), larger cases on the lanes between.
+        {
+            std::vector<io::Matrix> mix;
+            for (unsigned i = 0; i < 45; ++i)
+            {
+                const std::size_t m = 8 + (i * 37) % 249;
+                const std::size_t n = std::min<std::size_t>(m, 4 + (i * 53) % 125);
+                mix.push_back(gaussian(m, n, 5000 + i));
+                if (i == 11)
+                {
+                    mix.push_back(gaussian(300, 40, 77)); // beyond the batch rows: a lane
+                }
+                if (i == 30)
+                {
+                    mix.push_back(gaussian(260, 140, 78)); // beyond the batch columns: a lane
+                }
+            }
+            const fs::path bin = tmp / "batched.mat";
+            const fs::path bout = tmp / "batched_out.mat";
+            io::write_mat_file(bin, mix);
+            pipeline::PipelineConfig bc = pc;
+            bc.input_path = bin;
+            bc.output_path = bout;
+            bc.device_batch_cases = 16;
+            const pipeline::PipelineReport brep = pipeline::run_pipeline(bc);
+            const auto bres = io::read_mat_file(bout);
+            bool agree = bres.size() == 3 * mix.size() && brep.testcase_count == mix.size() && brep.total_sweeps > 0;
+            double worst_s = 0.0, worst_uv = 0.0;
+            for (std::size_t i = 0; agree && i < mix.size(); ++i)
+            {
+                const auto &u = bres[3 * i];
+                const auto &sg = bres[3 * i + 1];
+                const auto &v = bres[3 * i + 2];
+                agree = u.rows == mix[i].rows && u.columns == mix[i].columns && sg.columns == mix[i].columns &&
+                        v.rows == mix[i].columns;
+                if (!agree)
+                {
+                    break;
+                }
+                const JacobiSvdResult r = one_sided_jacobi_svd(mix[i].values, mix[i].rows, mix[i].columns,
+                                                               bc.kernel_config);
+                for (std::size_t j = 0; j < r.sigma.size(); ++j)
+                {
+                    worst_s = std::max(worst_s, std::abs(r.sigma[j] - sg.values[j]) / r.sigma[j]);
+                }
+                for (std::size_t j = 0; j < r.u.size(); ++j)
+                {
+                    worst_uv = std::max(worst_uv, std::abs(r.u[j] - u.values[j]));
+                }
+                for (std::size_t j = 0; j < r.v.size(); ++j)
+                {
+                    worst_uv = std::max(worst_uv, std::abs(r.v[j] - v.values[j]));
+                }
+            }
+            check(agree && worst_s < 1e-12 && worst_uv < 1e-9,
+                  "pipeline with device_batch_cases: input order kept, sigma within 1e-12 (worst " +
+                      std::to_string(worst_s) + "), U and V within 1e-9 of direct solves");
+        }
+
         // Pipeline txt -> txt.
         const fs::path tin = tmp / "cases.txt";
         const fs::path tout = tmp / "result.txt";

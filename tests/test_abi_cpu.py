@@ -72,3 +72,12 @@ def test_mat_helpers_match_hand_computed_record(tmp_path):
     open(p, "wb").write(golden[:-3])
     with pytest.raises(RuntimeError, match="payload"):
         jsvd.read_mat(p)
+
+
+def test_config_validate_without_a_matrix():
+    """jsvd_config_validate: the reference's validation messages, no matrix and no device needed."""
+    jsvd.validate_config(jsvd.JacobiSvdConfig())
+    with pytest.raises(ValueError, match="epsilon"):
+        jsvd.validate_config(jsvd.JacobiSvdConfig(epsilon=0.0))
+    with pytest.raises(ValueError, match="max_sweeps"):
+        jsvd.validate_config(jsvd.JacobiSvdConfig(max_sweeps=0))

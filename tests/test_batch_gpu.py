@@ -168,3 +168,27 @@ def test_batch_autotune_keeps_fastest_and_results(torch_cuda):
     assert b4.shape()[:2] == jsvd.BATCH_SHAPES[0]
     for x in (b, b2, b3, b4):
         x.close()
+
+
+def test_batch_host_entry_point_equals_device_entry_point(torch_cuda):
+    """jsvd_batch_solve_host (host buffers, staging inside the batch object) gives the device entry point's bits,
+    twice in a row (the second call with a smaller batch reuses the staging)."""
+    torch = torch_cuda
+    mats, prec, tol = _shape_mix()
+    d_items, d = _packed_device(torch, mats, prec, tol)
+    b = jsvd.Batch(256, 128)
+    b.solve_device(d_items, len(mats), *d)
+    ref = [x.cpu().numpy() for x in d[1:]]
+    pk = jsvd.BatchPacked(mats, prec, tol, 60)
+    u, s, v, sw = b.solve_host_packed(pk, order=pk.cost_order())
+    assert np.array_equal(u, ref[0]) and np.array_equal(s, ref[1]) and np.array_equal(v, ref[2])
+    assert np.array_equal(sw, ref[3])
+    small = jsvd.BatchPacked(mats[:5], prec[:5], tol[:5], 60)
+    u2, s2, v2, sw2 = b.solve_host_packed(small)
+    us, ss, vs, sws = b.solve(mats[:5], prec[:5], tol[:5], max_sweeps=60, order=np.arange(5))
+    for i in range(5):
+        it = small.items[i]
+        so, n = int(it["sigma_offset"]), int(it["columns"])
+        assert np.array_equal(s2[so:so + n], ss[i])
+    assert np.array_equal(sw2, sws)
+    b.close()

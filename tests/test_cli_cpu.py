@@ -61,6 +61,7 @@ def test_defaults_and_output_naming(tmp_path):
     assert (cfg["epsilon"], cfg["max-sweeps"], cfg["queue-capacity"]) == ("1e-12", "7", "2")
     assert (cfg["layout-mode"], cfg["layout-auto-tune"], cfg["force-overwrite"]) == ("off", "true", "true")
     assert dry(a, "--solver", "mixed")["solver"] == "mixed"
+    assert dry(a)["device-batch"] == "0" and dry(a, "--device-batch", "64")["device-batch"] == "64"
 
 
 @pytest.mark.parametrize("args,message", [
@@ -82,6 +83,8 @@ def test_defaults_and_output_naming(tmp_path):
     (("{d}/a.mat", "--epsilon"), "Option --epsilon requires a value."),
     (("{d}/a.mat", "{d}/b.mat", "{d}/c.mat"), "Unexpected positional argument"),
     (("{d}/a.mat", "{d}/b.txt", "--solver", "mixed"), "--solver block|mixed reads and writes *.mat files only."),
+    (("{d}/a.mat", "--device-batch", "0"), "Option --device-batch must be positive."),
+    (("{d}/a.mat", "--solver", "block", "--device-batch", "8"), "--device-batch applies to the plain solver only."),
 ])
 def test_argument_errors_exit_2(tmp_path, args, message):
     (tmp_path / "a.mat").write_bytes(b"")
