@@ -105,7 +105,8 @@ namespace jacobi::svd::pipeline
             throw std::runtime_error(message);
         }
 
-        // A solver lane: one HIP stream (owned by its plan) and the plan of the last shape it solved.
+        // A solver lane: one HIP stream (owned by its plan), a plan sized for the largest case it has seen, and
+        // (for batched groups) a batch object.
         class SolverLane
         {
         public:
@@ -133,21 +134,31 @@ namespace jacobi::svd::pipeline
                 }
                 const auto m = static_cast<int64_t>(a.rows);
                 const auto n = static_cast<int64_t>(a.columns);
-                if (!plan_ || m != m_ || n != n_)
+                // One plan per lane, sized for the largest case seen so far and re-targeted at each case
+                // (jsvd_plan_reshape: no allocation, results bit-equal to a plan of the exact shape).
+                if (!plan_ || m > m_ || n > n_)
                 {
                     if (plan_)
                     {
                         (void)jsvd_plan_destroy(plan_);
                         plan_ = nullptr;
                     }
-                    const int st = jsvd_plan_create(m, n, &cfg_, nullptr, &plan_);
+                    const int64_t cm = std::max(m, m_), cn = std::max(n, n_); // cm >= m >= n and cm >= m_ >= n_
+                    const int st = jsvd_plan_create(cm, cn, &cfg_, nullptr, &plan_);
                     if (st != JSVD_OK)
                     {
                         plan_ = nullptr;
                         raise(st);
                     }
-                    m_ = m;
-                    n_ = n;
+                    m_ = cm;
+                    n_ = cn;
+                }
+                {
+                    const int st = jsvd_plan_reshape(plan_, m, n, cfg_.epsilon);
+                    if (st != JSVD_OK)
+                    {
+                        raise(st);
+                    }
                 }
                 Packet p;
                 p.index = index;
