@@ -7,8 +7,8 @@
 //   mmap (page cache) --memcpy--> pinned staging[k % 2] --hipMemcpyAsync, copy stream--> raw HBM[k % 2]
 //   --decode kernel (big-endian u64 -> double), copy stream--> A[k % 2]   ... event ready[k % 2]
 //
-// The solver thread waits for the slot, solves it through a plan on the compute stream (plans are cached per
-// shape) into one of two result slots, and hands the slot to an output stream that encodes U, sigma (1 x n)
+// The solver thread waits for the slot, solves it through a plan on the compute stream (one plan per path,
+// re-targeted at each record's shape) into one of two result slots, and hands the slot to an output stream that encodes U, sigma (1 x n)
 // and V big-endian on the device and copies them to pinned memory; a writer thread appends the three records
 // to the output mapping in input order (the reference pipeline's output packets) while the next record is
 // solved.
@@ -18,7 +18,6 @@
 #include <condition_variable>
 #include <deque>
 #include <exception>
-#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -354,7 +353,13 @@ namespace jsvd
                     }
                 } writer_joiner{writer, mu, cv, writer_stop};
 
-                std::map<std::pair<std::size_t, std::size_t>, PlanPtr> plans;
+                struct PlanSlot
+                {
+                    PlanPtr plan;
+                    int64_t m = 0;
+                    int64_t n = 0;
+                };
+                PlanSlot plain_plan, path_plan;
                 for (std::size_t k = 0; k < records_.size(); ++k)
                 {
                     const int slot = static_cast<int>(k & 1);
@@ -377,19 +382,28 @@ namespace jsvd
                     const MatRecord &r = records_[k];
                     const int64_t m = static_cast<int64_t>(r.rows), n = static_cast<int64_t>(r.columns);
                     const auto ts = std::chrono::steady_clock::now();
-                    auto &plan = plans[{r.rows, r.columns}];
-                    if (!plan)
+                    // The block paths need two 32-column blocks; narrower records take the plain path.  One plan
+                    // per path, sized for the largest record it has met and re-targeted at each record
+                    // (jsvd_plan_reshape: no allocation per shape, results equal to a plan of the exact shape).
+                    const bool narrow = n < 33 || mixed_.path == JSVD_PATH_PLAIN_F64;
+                    PlanSlot &ps = narrow ? plain_plan : path_plan;
+                    if (!ps.plan || m > ps.m || n > ps.n)
                     {
-                        // The block paths need two 32-column blocks; narrower records take the plain path.
                         jsvd_mixed_config mx = mixed_;
-                        if (n < 33)
+                        if (narrow)
                         {
                             mx.path = JSVD_PATH_PLAIN_F64;
                         }
+                        const int64_t cm = std::max(m, ps.m), cn = std::max(n, ps.n); // cm >= cn
+                        ps.plan.reset();
                         jsvd_plan *p = nullptr;
-                        check_status(jsvd_plan_create_ex(m, n, &cfg_, &mx, compute.s, &p));
-                        plan.reset(p);
+                        check_status(jsvd_plan_create_ex(cm, cn, &cfg_, &mx, compute.s, &p));
+                        ps.plan.reset(p);
+                        ps.m = cm;
+                        ps.n = cn;
                     }
+                    check_status(jsvd_plan_reshape(ps.plan.get(), m, n, cfg_.epsilon));
+                    PlanPtr &plan = ps.plan;
                     JSVD_HIP_CHECK(hipStreamWaitEvent(compute.s, ready[slot].e, 0));
                     double *u = d_u[slot];
                     double *s = d_s[slot];
