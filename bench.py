@@ -438,6 +438,26 @@ def main() -> None:
         d = np.abs(np.sort(mixed["sigma"]) - np.sort(plain["sigma"]))
         mixed["accuracy"]["sigma_vs_plain_normwise"] = float(d.max() / np.max(plain["sigma"]))
 
+    reference_case = None
+    if args.workload == "tall" and not args.no_plain:
+        # the reference's own CPU-runnable case, 512 x 256 N(0,1), eps 1e-15, 30 sweeps, on both paths; sigma
+        # against LAPACK's (numpy.linalg.svd, well conditioned here) and between the paths
+        log("[bench] reference case 512x256")
+        c1, _, c1_eps, c1_sweeps, c1_name = make_workload("plain")
+        c1_cfg = jsvd.JacobiSvdConfig(epsilon=c1_eps, max_sweeps=c1_sweeps)
+        c1_plain = time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, c1, None, c1_cfg, 10, 1)
+        c1_mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, c1, None, c1_cfg, 10, 1)
+        s_ref = np.sort(np.linalg.svd(c1, compute_uv=False))[::-1]
+        sig = {}
+        for name, rep in (("plain", c1_plain), ("mixed", c1_mixed)):
+            sig[name] = np.sort(rep.pop("sigma"))[::-1]
+            rep["accuracy"]["sigma_relerr_vs_lapack"] = float(np.max(np.abs(sig[name] - s_ref) / s_ref))
+        reference_case = {"workload": c1_name, "plain_f64": c1_plain, "mixed": c1_mixed,
+                          "sigma_relerr_mixed_vs_plain": float(np.max(np.abs(sig["mixed"] - sig["plain"]) /
+                                                                      sig["plain"])),
+                          "speedup_mixed_vs_plain": c1_mixed["solves_per_s"] / c1_plain["solves_per_s"]}
+        log(f"[bench] reference case: plain {c1_plain['solves_per_s']:.1f}/s, mixed {c1_mixed['solves_per_s']:.1f}/s")
+
     file_stream = None
     if args.workload == "tall" and not args.no_file:
         log("[bench] streamed *.mat file path (mixed)")
@@ -526,6 +546,7 @@ def main() -> None:
         "mixed": mixed,
         "plain_f64": plain,
         "speedup_vs_plain": (mixed["solves_per_s"] / plain["solves_per_s"]) if plain else None,
+        "reference_case_512x256": reference_case,
         "file_stream_mixed": file_stream,
         "batched": batch,
         "serving": serving,
