@@ -2,8 +2,8 @@
 // FP64 matrix, used by the mixed path's Cholesky QR.
 //
 // Right-looking over 64 x 64 tiles, three launches per tile step k:
-//   cholesky_diag_kernel    one wavefront: factors the diagonal tile A_kk with its columns in registers (64
-//                           steps: pivot and scaled row broadcast across lanes, rank-1 update) and writes R_kk
+//   cholesky_diag_kernel    four wavefronts: factor the diagonal tile A_kk with its columns in registers (64
+//                           steps: pivot, scaled row through LDS, rank-1 update) and write R_kk
 //   cholesky_panel_kernel   one wavefront per tile column j > k: solves R_kk^T R_kj = A_kj by forward
 //                           substitution, each lane one column in registers
 //   cholesky_update_kernel  one workgroup per trailing tile (i, j), k < i <= j: A_ij -= R_ki^T R_kj on FP64 MFMA
@@ -28,46 +28,75 @@ namespace jsvd
             return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
         }
 
-        // One wavefront: lane l holds column l of the diagonal tile in registers (x[r] = A[r][l]).  Step c reads
-        // the pivot and the scaled row c (R[c][i] from lane i) with v_readlane and updates every lane's column;
-        // the lanes' strictly lower entries take part in the arithmetic but are never written back.
-        __global__ __launch_bounds__(64) void cholesky_diag_kernel(double *__restrict__ a, int64_t ld, int k,
-                                                                   int *__restrict__ info)
+        constexpr int kDiagWaves = 4;               // wavefronts of the diagonal-tile kernel
+        constexpr int kDiagRows = kT / kDiagWaves;  // tile rows each of them holds
+
+        // Four wavefronts: thread (w, l) holds rows 16 w .. 16 w + 15 of column l of the diagonal tile in
+        // registers.  Step c: the wavefront owning row c reads the pivot from lane c (v_readlane), forms the
+        // scaled row R[c][l] = A[c][l] / sqrt(pivot) and publishes it in LDS (two buffers, alternating, so one
+        // barrier per step suffices); every wavefront then applies the rank-1 update to its rows below c,
+        // reading R[c][i] as LDS broadcasts.  A lane's entries below its column's diagonal take part in the
+        // arithmetic but are never written back.
+        __global__ __launch_bounds__(kT * kDiagWaves) void cholesky_diag_kernel(double *__restrict__ a, int64_t ld,
+                                                                               int k, int *__restrict__ info)
         {
-            const int l = threadIdx.x;
+            __shared__ __align__(16) double row[2][kT];
+            const int l = threadIdx.x % kT;
+            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kT)); // wave-uniform
             const int64_t r0 = static_cast<int64_t>(k) * kT;
-            double *col = a + (r0 + l) * ld + r0;
-            double x[kT];
+            double *col = a + (r0 + l) * ld + r0 + w * kDiagRows;
+            double x[kDiagRows];
 #pragma unroll
-            for (int r = 0; r < kT; ++r)
+            for (int i = 0; i < kDiagRows; ++i)
             {
-                x[r] = col[r];
+                x[i] = col[i];
             }
             bool bad = false;
 #pragma unroll
             for (int c = 0; c < kT; ++c)
             {
-                double piv = lane_value(x[c], c);
-                if (!(piv > 0.0))
+                const int b = c & 1;
+                const int ow = c / kDiagRows; // owner wavefront and its register of row c (compile-time)
+                const int oi = c % kDiagRows;
+                if (w == ow)
                 {
-                    bad = true;
-                    piv = 1.0; // keep the arithmetic finite; the factorisation is reported as failed
+                    double piv = lane_value(x[oi], c);
+                    if (!(piv > 0.0))
+                    {
+                        bad = true;
+                        piv = 1.0; // keep the arithmetic finite; the factorisation is reported as failed
+                    }
+                    const double r = sqrt(piv);
+                    const double u = x[oi] / r; // R[c][l] for l > c
+                    x[oi] = (l == c) ? r : u;
+                    row[b][l] = u;
                 }
-                const double r = sqrt(piv);
-                const double u = x[c] / r; // R[c][l] for l > c
-                x[c] = (l == c) ? r : u;
-#pragma unroll
-                for (int i = c + 1; i < kT; ++i)
+                __syncthreads();
+                const double ul = row[b][l];
+                if (w > ow)
                 {
-                    x[i] = fma(-lane_value(u, i), u, x[i]);
+                    const double *rc = &row[b][w * kDiagRows];
+#pragma unroll
+                    for (int i = 0; i < kDiagRows; ++i)
+                    {
+                        x[i] = fma(-rc[i], ul, x[i]);
+                    }
+                }
+                else if (w == ow)
+                {
+#pragma unroll
+                    for (int i = oi + 1; i < kDiagRows; ++i)
+                    {
+                        x[i] = fma(-row[b][ow * kDiagRows + i], ul, x[i]);
+                    }
                 }
             }
 #pragma unroll
-            for (int r = 0; r < kT; ++r)
+            for (int i = 0; i < kDiagRows; ++i)
             {
-                if (r <= l)
+                if (w * kDiagRows + i <= l)
                 {
-                    col[r] = x[r];
+                    col[i] = x[i];
                 }
             }
             if (l == 0 && bad)
@@ -219,7 +248,7 @@ namespace jsvd
         const int nt = static_cast<int>(n64 / kT);
         for (int k = 0; k < nt; ++k)
         {
-            hipLaunchKernelGGL(cholesky_diag_kernel, dim3(1), dim3(64), 0, stream, a, ld, k, info);
+            hipLaunchKernelGGL(cholesky_diag_kernel, dim3(1), dim3(kT * kDiagWaves), 0, stream, a, ld, k, info);
             JSVD_HIP_CHECK(hipGetLastError());
             const int rest = nt - k - 1;
             if (rest > 0)
