@@ -4,8 +4,8 @@
 // Right-looking over 64 x 64 tiles, three launches per tile step k:
 //   cholesky_diag_kernel    four wavefronts: factor the diagonal tile A_kk with its columns in registers (64
 //                           steps: pivot, scaled row through LDS, rank-1 update) and write R_kk
-//   cholesky_panel_kernel   one wavefront per tile column j > k: solves R_kk^T R_kj = A_kj by forward
-//                           substitution, each lane one column in registers
+//   cholesky_panel_kernel   four wavefronts per tile column j > k: solve R_kk^T R_kj = A_kj by forward
+//                           substitution, each column held by a quad of lanes (16 rows each)
 //   cholesky_update_kernel  one workgroup per trailing tile (i, j), k < i <= j: A_ij -= R_ki^T R_kj on FP64 MFMA
 //                           (v_mfma_f64_16x16x4f64), both operand tiles staged in LDS
 // Only the upper triangle is read and written; the strictly lower triangle keeps whatever it held.
@@ -105,45 +105,92 @@ namespace jsvd
             }
         }
 
-        // Tile column j = k + 1 + blockIdx.x of block row k, one wavefront: lane c holds column c of the tile in
-        // registers and solves R_kk^T y = x by forward substitution (y[r] = x[r] / R[r][r], then x[p] -= R[r][p]
-        // y[r] for p > r), R_kk staged in LDS and read as broadcasts.
-        __global__ __launch_bounds__(64) void cholesky_panel_kernel(double *__restrict__ a, int64_t ld, int k)
+        constexpr int kPanelWaves = 4;             // wavefronts of the panel kernel
+        constexpr int kPanelRows = kT / 4;         // rows of a column each lane of a quad holds
+        constexpr int kRzStride = kT + 2;          // LDS row stride of R_kk (16-byte aligned pairs)
+
+        // Broadcast of v from lane `src` of every quad (DPP quad_perm), under full exec.
+        template <int SRC>
+        __device__ inline double quad_broadcast(double v)
         {
-            __shared__ double rt[kT][kT]; // rt[p][r] = R_kk[r][p] (column p of R_kk), upper part read
-            __shared__ double inv[kT];    // 1 / R_kk[r][r]
+            constexpr int kCtrl = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6);
+            const long long bits = __double_as_longlong(v);
+            int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(bits), kCtrl, 0xF, 0xF, false);
+            int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(bits >> 32), kCtrl, 0xF, 0xF, false);
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+        }
+
+        // Tile column j = k + 1 + blockIdx.x of block row k, four wavefronts: each quad of lanes holds one
+        // column of the tile (lane q of the quad rows 16 q .. 16 q + 15) and solves R_kk^T y = x by forward
+        // substitution: step r, the lane holding row r forms y[r] = x[r] / R[r][r] (times the reciprocal), the
+        // quad shares it by a DPP broadcast and every lane applies x[p] -= R[r][p] y[r] to its rows.  R_kk sits in
+        // LDS with its entries on and below the diagonal replaced by zeros, so rows p <= r take a zero update and
+        // no lane needs a per-row test; the arithmetic of every entry above is that of the one-lane-per-column
+        // substitution.
+        __global__ __launch_bounds__(kT * kPanelWaves) void cholesky_panel_kernel(double *__restrict__ a, int64_t ld,
+                                                                                 int k)
+        {
+            __shared__ __align__(16) double rz[kT][kRzStride]; // rz[r][p] = R_kk[r][p] for p > r, else 0
+            __shared__ double inv[kT];                          // 1 / R_kk[r][r]
+            const int tid = threadIdx.x;
             const int j = k + 1 + static_cast<int>(blockIdx.x);
-            const int lane = threadIdx.x;
             const int64_t r0 = static_cast<int64_t>(k) * kT;
             const double *akk = a + r0 * ld + r0;
-            for (int p = 0; p < kT; ++p)
+            // all loads of R_kk and of this lane's column in flight at once, then the LDS stores
+            constexpr int kLoads = kT * kT / (kT * kPanelWaves);
+            double v[kLoads];
+#pragma unroll
+            for (int t = 0; t < kLoads; ++t)
             {
-                rt[p][lane] = akk[static_cast<int64_t>(p) * ld + lane];
+                const int e = tid + t * kT * kPanelWaves;
+                v[t] = akk[static_cast<int64_t>(e / kT) * ld + e % kT]; // column e / kT, row e % kT
             }
-            double *col = a + (static_cast<int64_t>(j) * kT + lane) * ld + r0;
-            double x[kT];
+            const int lane = tid % kT;
+            const int q = lane & 3;                             // row quarter of this lane
+            const int c = (tid / kT) * (kT / 4) + (lane >> 2);  // tile column of this quad
+            double *col = a + (static_cast<int64_t>(j) * kT + c) * ld + r0 + q * kPanelRows;
+            double x[kPanelRows];
 #pragma unroll
-            for (int r = 0; r < kT; ++r)
+            for (int i = 0; i < kPanelRows; ++i)
             {
-                x[r] = col[r];
+                x[i] = col[i];
             }
-            __syncthreads();
-            inv[lane] = 1.0 / rt[lane][lane];
-            __syncthreads();
 #pragma unroll
-            for (int r = 0; r < kT; ++r)
+            for (int t = 0; t < kLoads; ++t)
             {
-                x[r] *= inv[r];
-#pragma unroll
-                for (int p = r + 1; p < kT; ++p)
+                const int e = tid + t * kT * kPanelWaves;
+                const int p = e / kT;
+                const int r = e % kT;
+                rz[r][p] = (p > r) ? v[t] : 0.0;
+                if (p == r)
                 {
-                    x[p] = fma(-rt[p][r], x[r], x[p]);
+                    inv[r] = 1.0 / v[t];
+                }
+            }
+            __syncthreads();
+            const double *rq = &rz[0][q * kPanelRows];
+#pragma unroll
+            for (int r = 0; r < kT; ++r)
+            {
+                const int owner = r / kPanelRows; // compile-time after unrolling
+                const int ir = r % kPanelRows;
+                const double y = x[ir] * inv[r];
+                x[ir] = (q == owner) ? y : x[ir];
+                const double yb = (owner == 0)   ? quad_broadcast<0>(y)
+                                  : (owner == 1) ? quad_broadcast<1>(y)
+                                  : (owner == 2) ? quad_broadcast<2>(y)
+                                                 : quad_broadcast<3>(y);
+#pragma unroll
+                for (int i = 0; i < kPanelRows; ++i)
+                {
+                    x[i] = fma(-rq[r * kRzStride + i], yb, x[i]);
                 }
             }
 #pragma unroll
-            for (int r = 0; r < kT; ++r)
+            for (int i = 0; i < kPanelRows; ++i)
             {
-                col[r] = x[r];
+                col[i] = x[i];
             }
         }
 
@@ -162,15 +209,39 @@ namespace jsvd
             const int64_t r0 = static_cast<int64_t>(k) * kT;
             const double *pi = a + static_cast<int64_t>(i) * kT * ld + r0;
             const double *pj = a + static_cast<int64_t>(j) * kT * ld + r0;
-            for (int e = tid; e < kT * kT; e += 256)
+            // all loads of both operand tiles and of the output tile in flight at once
+            constexpr int kLoads = kT * kT / 256;
+            double vi[kLoads], vj[kLoads];
+#pragma unroll
+            for (int t = 0; t < kLoads; ++t)
             {
+                const int e = tid + t * 256;
                 const int c = e / kT, r = e % kT;
-                ri[c][r] = pi[static_cast<int64_t>(c) * ld + r];
-                rj[c][r] = pj[static_cast<int64_t>(c) * ld + r];
+                vi[t] = pi[static_cast<int64_t>(c) * ld + r];
+                vj[t] = pj[static_cast<int64_t>(c) * ld + r];
             }
-            __syncthreads();
             const int lane = tid % 64;
             const int wave = tid / 64;
+            double *out = a + static_cast<int64_t>(j) * kT * ld + static_cast<int64_t>(i) * kT;
+            double prev[4][4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    prev[t][v] = out[static_cast<int64_t>(16 * t + lane % 16) * ld + 16 * wave + lane / 16 + 4 * v];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kLoads; ++t)
+            {
+                const int e = tid + t * 256;
+                const int c = e / kT, r = e % kT;
+                ri[c][r] = vi[t];
+                rj[c][r] = vj[t];
+            }
+            __syncthreads();
             // C = R_ki^T R_kj: wave w computes rows 16w..16w+15 of C, all four 16-column tiles.
             f64x4c acc[4];
 #pragma unroll
@@ -189,7 +260,6 @@ namespace jsvd
                     acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
                 }
             }
-            double *out = a + static_cast<int64_t>(j) * kT * ld + static_cast<int64_t>(i) * kT;
 #pragma unroll
             for (int t = 0; t < 4; ++t)
             {
@@ -200,7 +270,7 @@ namespace jsvd
                     const int row = 16 * wave + lane / 16 + 4 * v;
                     if (i < j || row <= col) // a diagonal tile keeps its strictly lower triangle
                     {
-                        out[static_cast<int64_t>(col) * ld + row] -= acc[t][v];
+                        out[static_cast<int64_t>(col) * ld + row] = prev[t][v] - acc[t][v];
                     }
                 }
             }
@@ -253,8 +323,8 @@ namespace jsvd
             const int rest = nt - k - 1;
             if (rest > 0)
             {
-                hipLaunchKernelGGL(cholesky_panel_kernel, dim3(static_cast<unsigned>(rest)), dim3(64), 0, stream, a,
-                                   ld, k);
+                hipLaunchKernelGGL(cholesky_panel_kernel, dim3(static_cast<unsigned>(rest)), dim3(kT * kPanelWaves),
+                                   0, stream, a, ld, k);
                 JSVD_HIP_CHECK(hipGetLastError());
                 hipLaunchKernelGGL(cholesky_update_kernel, dim3(static_cast<unsigned>(rest), static_cast<unsigned>(rest)),
                                    dim3(256), 0, stream, a, ld, k);
