@@ -199,6 +199,14 @@ int jsvd_cholesky_f64(double *a, int64_t lda, int64_t n, int32_t *info, void *st
 #define JSVD_PRECISION_F64 0
 #define JSVD_PRECISION_F32 1
 
+/* Smallest column count a batch item may have.  Every item needs JSVD_BATCH_MIN_COLUMNS <= columns <=
+ * max_columns and columns <= rows <= max_rows (the batch object's limits), max_sweeps >= 1, tolerance > 0, a
+ * known precision and offsets inside the packed buffers.  The host-pointer entry point checks every item and
+ * fails with JSVD_ERR_INVALID_ARGUMENT before anything is launched; for the device-pointer entry points this is
+ * the caller's precondition (the kernel still writes sweeps = -1 for an item outside the limits and skips it).
+ * Matrices with fewer columns go through the plan entry points (jsvd_plan_*). */
+#define JSVD_BATCH_MIN_COLUMNS 4
+
 /* One matrix of a batch.  Offsets are element offsets into the batch's packed FP64 buffers: A row-major
  * rows x columns at a_offset, U row-major rows x columns at u_offset, sigma (columns) at sigma_offset,
  * V row-major columns x columns at v_offset.  tolerance is the matrix's epsilon. */

@@ -256,10 +256,11 @@ namespace jacobi::svd::pipeline
             int64_t n_ = 0;
         };
 
-        // Cases the batched launch takes: well-formed, rows >= columns, within the batch limits.
+        // Cases the batched launch takes: well-formed, rows >= columns >= JSVD_BATCH_MIN_COLUMNS, within the batch
+        // limits.  Narrower cases go through the lane's plan.
         bool batchable(std::size_t rows, std::size_t columns)
         {
-            return rows >= 1 && columns >= 1 && rows >= columns &&
+            return columns >= static_cast<std::size_t>(JSVD_BATCH_MIN_COLUMNS) && rows >= columns &&
                    rows <= static_cast<std::size_t>(SolverLane::kBatchRows) &&
                    columns <= static_cast<std::size_t>(SolverLane::kBatchColumns);
         }

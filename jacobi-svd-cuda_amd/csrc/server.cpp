@@ -517,7 +517,8 @@ int jsvd_server_submit(jsvd_server *server, const double *a, int64_t m, int64_t 
         r->tol = (r->precision == JSVD_PRECISION_F32)
                      ? 1e-6
                      : std::max(1e-14, 2.0 * std::sqrt(static_cast<double>(m)) * std::ldexp(1.0, -53));
-        r->small = m <= server->cfg.small_max_rows && n <= server->cfg.small_max_columns;
+        r->small = m <= server->cfg.small_max_rows && n <= server->cfg.small_max_columns &&
+                   n >= JSVD_BATCH_MIN_COLUMNS;
         r->bytes = jsvd_server::footprint(m, n);
         r->a.assign(a, a + m * n);
         r->submitted = Clock::now();

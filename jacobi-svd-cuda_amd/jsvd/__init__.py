@@ -455,6 +455,7 @@ def block_eigen(gram, eps: float, q, rotations, stream=None) -> None:
 # batched path (jsvd_batch_*)
 # ------------------------------------------------------------------------------------------------
 PRECISION_F64, PRECISION_F32 = 0, 1
+BATCH_MIN_COLUMNS = 4  # JSVD_BATCH_MIN_COLUMNS (include/jsvd.h)
 
 BATCH_ITEM_DTYPE = np.dtype([
     ("a_offset", "<i8"), ("u_offset", "<i8"), ("sigma_offset", "<i8"), ("v_offset", "<i8"),
@@ -536,6 +537,15 @@ class BatchPacked:
             self.a[o:o + a.size] = np.ascontiguousarray(a, dtype=np.float64).ravel()
         self.lens = (a_off, u_off, s_off, v_off)
 
+    def check_limits(self, max_rows: int, max_cols: int):
+        """The batch item contract (JSVD_BATCH_MIN_COLUMNS <= n <= max_cols, n <= m <= max_rows); ValueError
+        naming the first item outside it."""
+        for i, it in enumerate(self.items):
+            m, n = int(it["rows"]), int(it["columns"])
+            if not (BATCH_MIN_COLUMNS <= n <= max_cols and n <= m <= max_rows):
+                raise ValueError(f"batch item {i}: needs {BATCH_MIN_COLUMNS} <= columns <= rows within the batch "
+                                 f"limits ({max_rows} x {max_cols}), got {m} x {n}")
+
     def cost_order(self):
         """Queue order: most expensive first (m n^2), stable."""
         cost = self.items["rows"].astype(np.float64) * self.items["columns"].astype(np.float64) ** 2
@@ -552,6 +562,7 @@ class Batch:
         s = _stream_handle(stream) if stream is not None else None
         check(L.jsvd_batch_create(int(max_rows), int(max_cols), s, ctypes.byref(h)))
         self._h = h
+        self.max_rows, self.max_cols = int(max_rows), int(max_cols)
 
     def close(self):
         if self._h:
@@ -614,6 +625,7 @@ class Batch:
         sweeps array) in input order."""
         import torch
         pk = BatchPacked(mats, precisions, tolerances, max_sweeps)
+        pk.check_limits(self.max_rows, self.max_cols)
         if order is None:
             order = pk.cost_order()
         items = pk.items[order]
