@@ -272,6 +272,25 @@ int jsvd_batch_solve_host(jsvd_batch *batch, const jsvd_batch_item *items, int64
                           int64_t v_len, int32_t *sweeps);
 
 /*
+ * Allocation-free form of jsvd_batch_solve_async.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No batch object: the same persistent kernel runs with its queue counter and workgroup scratch
+ * slices in a fixed device array of 128 MiB inside the library (one copy per GPU, held from the first call on
+ * that GPU until the process ends); the caller passes the item descriptors and packed buffers in device memory
+ * (items under the contract of JSVD_BATCH_MIN_COLUMNS), the limits max_rows x max_columns its items stay within,
+ * and a stream.  Nothing is allocated and nothing is waited for: the work is enqueued on `stream` (a
+ * hipStream_t; null = the default stream) and the call returns.  The limits fix the scratch slice per
+ * workgroup, ((max_rows + 1) + (max_columns + 1)) * max_columns doubles; limits whose slice does not fit the
+ * array are refused with JSVD_ERR_INVALID_ARGUMENT, and the number of resident workgroups is capped by the
+ * slices it holds (e.g. 339 at 256 x 128).  The launch shape is the one jsvd_batch_autotune recorded for these
+ * limits, else the default.  Results are bit-identical to jsvd_batch_solve_device's.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (on different streams, or from different threads without
+ * ordering) race on it and give undefined results; calls ordered on one stream are safe.
+ */
+int jsvd_batch_solve_noalloc(const jsvd_batch_item *d_items, int64_t count, const jsvd_batch_buffers *buf,
+                             int64_t max_rows, int64_t max_columns, void *stream);
+
+/*
  * Launch shape of the persistent batched kernel (no reference counterpart): threads per workgroup (the
  * wavefronts sharing one matrix's column pairs) and the LDS bytes each workgroup stages its matrix in.  The
  * shapes are (256, 77824) (the default), (128, 38912), (256, 38912), (512, 77824), (512, 155648) and

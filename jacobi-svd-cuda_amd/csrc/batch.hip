@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "autotune.hpp"
 #include "batch_kernel.hpp"
@@ -201,6 +202,60 @@ namespace
     }
 } // namespace
 
+// Device memory of the allocation-free entry point (jsvd_batch_solve_noalloc): the first 256 bytes hold the
+// persistent kernel's queue counter, the rest one scratch slice per resident workgroup.  Every GPU the library
+// runs on holds its own copy from the first use until the process ends.  Host code reaches it only through the
+// per-device address from hipGetSymbolAddress (static_scratch_address below), never through the name.
+constexpr int64_t kStaticScratchBytes = int64_t(128) << 20;
+constexpr int64_t kStaticHeaderDoubles = 256 / sizeof(double);
+static __device__ __align__(256) double scratch[kStaticScratchBytes / sizeof(double)];
+
+namespace
+{
+    __global__ void clear_queue_counter_kernel(unsigned long long *counter)
+    {
+        *counter = 0ull; // a kernel, not hipMemsetAsync, so a caller may capture the call in a graph
+    }
+
+    constexpr int kStaticMaxDevices = 64;
+    std::mutex g_static_mu;
+    void *g_static_address[kStaticMaxDevices] = {};
+    int g_static_per_cu[kStaticMaxDevices][JSVD_BATCH_MAX_SHAPES] = {};
+    int g_static_cus[kStaticMaxDevices] = {};
+
+    // Device address of `scratch` on the current device (looked up once per device, with that device current),
+    // plus the resident workgroups per CU of launch shape `shape` and the CU count.  Allocates nothing.
+    double *static_scratch_address(int shape, int &per_cu, int &cus)
+    {
+        int device = 0;
+        JSVD_HIP_CHECK(hipGetDevice(&device));
+        if (device < 0 || device >= kStaticMaxDevices)
+        {
+            jsvd::throw_invalid("device index beyond the allocation-free entry point's table.");
+        }
+        std::lock_guard<std::mutex> lock(g_static_mu);
+        if (!g_static_address[device])
+        {
+            void *ptr = nullptr;
+            JSVD_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            int n_cus = 0;
+            JSVD_HIP_CHECK(hipDeviceGetAttribute(&n_cus, hipDeviceAttributeMultiprocessorCount, device));
+            g_static_cus[device] = std::max(n_cus, 1);
+            g_static_address[device] = ptr;
+        }
+        if (g_static_per_cu[device][shape] == 0)
+        {
+            int occ = 0;
+            JSVD_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ, reinterpret_cast<const void *>(jsvd::kShapes[shape].kernel), jsvd::kShapes[shape].threads, 0));
+            g_static_per_cu[device][shape] = std::max(occ, 1);
+        }
+        per_cu = g_static_per_cu[device][shape];
+        cus = g_static_cus[device];
+        return static_cast<double *>(g_static_address[device]);
+    }
+} // namespace
+
 extern "C" {
 
 int jsvd_batch_create(int64_t max_rows, int64_t max_columns, void *stream, jsvd_batch **batch)
@@ -276,6 +331,50 @@ int jsvd_batch_solve_device(jsvd_batch *batch, const jsvd_batch_item *d_items, i
         return st;
     }
     return jsvd::guarded([&] { JSVD_HIP_CHECK(hipStreamSynchronize(batch->stream)); });
+}
+
+int jsvd_batch_solve_noalloc(const jsvd_batch_item *d_items, int64_t count, const jsvd_batch_buffers *buf,
+                             int64_t max_rows, int64_t max_columns, void *stream)
+{
+    return jsvd::guarded([&] {
+        check_buffers(buf, count);
+        if (count > 0 && !d_items)
+        {
+            jsvd::throw_invalid("items must not be null.");
+        }
+        if (max_columns < 1 || max_rows < max_columns || max_rows > 4096)
+        {
+            jsvd::throw_invalid("batch limits need 1 <= max_columns <= max_rows <= 4096.");
+        }
+        // one scratch slice per resident workgroup, as jsvd_batch_create sizes them
+        const int64_t slice = (max_rows + 1) * max_columns + (max_columns + 1) * max_columns;
+        const int64_t slices = (kStaticScratchBytes / int64_t(sizeof(double)) - kStaticHeaderDoubles) / slice;
+        if (slices < 1)
+        {
+            jsvd::throw_invalid("batch limits " + std::to_string(max_rows) + " x " + std::to_string(max_columns) +
+                                " need a scratch slice of " + std::to_string(slice * sizeof(double)) +
+                                " bytes per workgroup; the allocation-free entry point holds " +
+                                std::to_string(kStaticScratchBytes - 256) + " bytes.");
+        }
+        if (count == 0)
+        {
+            return;
+        }
+        jsvd::Tuning t;
+        const int shape = (jsvd::lookup_tuning(max_rows, max_columns, t) && t.has_batch) ? t.batch_shape : 0;
+        int per_cu = 1, cus = 1;
+        double *base = static_scratch_address(shape, per_cu, cus);
+        const int64_t grid = std::min<int64_t>(std::min<int64_t>(int64_t(per_cu) * cus, slices), count);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        auto *counter = reinterpret_cast<unsigned long long *>(base);
+        hipLaunchKernelGGL(clear_queue_counter_kernel, dim3(1), dim3(1), 0, s, counter);
+        JSVD_HIP_CHECK(hipGetLastError());
+        const jsvd::BatchShape &sh = jsvd::kShapes[shape];
+        hipLaunchKernelGGL(sh.kernel, dim3(static_cast<unsigned>(grid)), dim3(sh.threads), 0, s, d_items, count, *buf,
+                           counter, base + kStaticHeaderDoubles, slice, static_cast<int>(max_rows),
+                           static_cast<int>(max_columns));
+        JSVD_HIP_CHECK(hipGetLastError());
+    });
 }
 
 int jsvd_batch_set_shape(jsvd_batch *batch, int32_t threads, int32_t lds_bytes)

@@ -486,7 +486,22 @@ def _bind_batch(L):
     L.jsvd_batch_autotune.argtypes = [_vp, _vp, _i64, ctypes.POINTER(BatchBuffers), ctypes.c_int32,
                                       ctypes.POINTER(BatchTuneReport)]
     L.jsvd_batch_solve_host.argtypes = [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]
+    L.jsvd_batch_solve_noalloc.argtypes = [_vp, _i64, ctypes.POINTER(BatchBuffers), _i64, _i64, _vp]
     L._batch_bound = True
+
+
+def batch_solve_noalloc(d_items, count, d_a, d_u, d_s, d_v, d_sweeps, max_rows: int, max_cols: int, stream=None):
+    """jsvd_batch_solve_noalloc: the batched solve with the library's fixed device scratch (no allocation, no
+    batch object).  Device tensors as for Batch.solve_device; enqueued on `stream` (a torch stream, default the
+    current one) without waiting."""
+    import torch
+    L = lib()
+    _bind_batch(L)
+    buf = BatchBuffers(d_a.data_ptr(), d_u.data_ptr(), d_s.data_ptr(), d_v.data_ptr(), d_sweeps.data_ptr(),
+                       d_a.numel(), d_u.numel(), d_s.numel(), d_v.numel())
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(L.jsvd_batch_solve_noalloc(d_items.data_ptr(), int(count), ctypes.byref(buf), int(max_rows), int(max_cols),
+                                     _stream_handle(s)))
 
 
 BATCH_MAX_SHAPES = 8
