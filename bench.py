@@ -83,7 +83,9 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
     return {
         "kernel": "block_eigen_kernel<float> (Gram-space inner sweep of a block round: 63 rounds of 2x2 updates "
                   "of G and Q, rotation parameters in FP64; one workgroup per block pair)",
-        "bound": "mfma",
+        "bound": "latency",
+        "bound_detail": "dependent chain of LDS reads, VALU updates and a barrier per inner round; issues no MFMA "
+                        "and moves 0.6 MB per launch, so neither the matrix nor the HBM roof binds it",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -168,35 +170,45 @@ def round_kernel_hbm_roofline(torch, jsvd, m=16384, n=4096, launches=24):
             "rotated_pairs": rotated, "pairs": pairs}
 
 
-def device_copy_bandwidth(torch, mib=512, reps=5):
-    """Achievable HBM bandwidth of a plain device-to-device copy (torch copy_, read + write bytes) on a buffer
-    twice the 256 MiB MALL, timed with events on the current stream: the practical roof for kernels that
-    read and write the same number of bytes, beside the 8 TB/s peak."""
+def device_copy_bandwidth(torch, jsvd, mib=512, reps=5):
+    """Achievable HBM bandwidth of a device-to-device copy (read + write bytes) on a buffer twice the 256 MiB MALL,
+    by the library's own streaming copy kernel (jsvd_hbm_copy, plain and non-temporal stores; the faster one is the
+    practical roof for kernels that read and write the same number of bytes) and by torch's copy_ for comparison,
+    each timed with HIP events on the stream it runs on."""
     src = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
     stream = torch.cuda.current_stream()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0.record(stream)
-    for _ in range(reps):
-        dst.copy_(src)
-    t1.record(stream)
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / reps
-    gbs = 2 * src.numel() * 8 / (ms * 1e-3) / 1e9
+    nbytes = 2 * src.numel() * 8
+    out = {"bytes_per_copy": nbytes}
+    runs = {"hbm_copy_kernel": lambda: jsvd.hbm_copy(dst, src, 0, stream=stream),
+            "hbm_copy_kernel_nt": lambda: jsvd.hbm_copy(dst, src, 1, stream=stream),
+            "torch_copy": lambda: dst.copy_(src)}
+    for name, fn in runs.items():
+        fn()
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for _ in range(reps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        out[name] = {"avg_us": ms * 1e3, "achieved_gbs": nbytes / (ms * 1e-3) / 1e9}
+    assert torch.equal(dst, src)
+    best = max(out["hbm_copy_kernel"]["achieved_gbs"], out["hbm_copy_kernel_nt"]["achieved_gbs"])
+    out["achieved_gbs"] = best
+    out["frac_of_peak"] = best / HBM_PEAK_GBS
     del src, dst
-    return {"bytes_per_copy": 2 * mib * 1024 * 1024, "avg_us": ms * 1e3, "achieved_gbs": gbs,
-            "frac_of_peak": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
     """Batched persistent launch over `count` uneven matrices (m ~ U{8..256}, n ~ U{4..min(m,128)}, entries
     N(0,1) x 10^U(-3,3), alternate items FP32 tol 1e-6 / FP64 tol 1e-14), inputs resident in HBM, queue in cost
     order.  The launch shape is measured by jsvd_batch_autotune first unless autotuning is off.  Accuracy of a
-    sample of items against LAPACK (numpy.linalg.svd): FP64 items sigma relative error, FP32 items sigma error
-    relative to sigma_max."""
+    sample of items against LAPACK gesdd (numpy.linalg.svd; these matrices are well conditioned): FP64 items sigma
+    relative error, FP32 items sigma error relative to sigma_max."""
     from jsvd import workloads
     items = workloads.batch_mixed(count)
     mats = [a for a, _, _ in items]
@@ -246,8 +258,8 @@ def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
         "autotune": tune,
         "sweeps_mean": float(sweeps.mean()),
         "rejected": int(np.sum(sweeps < 0)),
-        "accuracy_sample": {"items": min(checked, count), "f64_sigma_relerr_max": worst64,
-                            "f32_sigma_err_rel_sigma_max": worst32},
+        "accuracy_sample": {"items": min(checked, count), "against": "LAPACK gesdd (numpy.linalg.svd)",
+                            "f64_sigma_relerr_max": worst64, "f32_sigma_err_rel_sigma_max": worst32},
     }
 
 
@@ -282,6 +294,19 @@ def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=4):
     }
 
 
+def lapack_dgejsv_sigma(a):
+    """Singular values (descending) of a by LAPACK dgejsv, the preconditioned one-sided Jacobi SVD (SciPy's
+    bundled LAPACK), or None where SciPy is missing."""
+    try:
+        from scipy.linalg import lapack
+    except ImportError:
+        return None
+    sva, _, _, work, _, info = lapack.dgejsv(np.asarray(a, dtype=np.float64), jobu=3, jobv=3)
+    if info != 0:
+        return None
+    return np.sort(sva * (work[1] / work[0]))[::-1]
+
+
 def profiled_traffic(key):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (profiles/traffic.md: lines
     `traffic <key> <bytes>`, FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), or None."""
@@ -295,28 +320,80 @@ def profiled_traffic(key):
     return None
 
 
-def cpu_baseline(a_host, eps, gpu_sweeps, sample_sweeps):
+def host_description():
+    """CPU model string, logical CPUs usable by this process and OMP_NUM_THREADS of the host running the bench."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": usable, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(a_host, eps, max_sweeps, reps=3, c1_reps=5):
+    """The oracle (C restatement of the reference algorithm: same pair order, rotation, skip test and convergence
+    rule; OpenMP over the disjoint pairs of a round, a barrier per round) timed on this host's cores: `reps` full
+    solves of the workload (median), and the reference's own 512 x 256 case (one warm-up, median of `c1_reps`
+    full solves) with LAPACK dgesvj beside it on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = os.cpu_count() or 1
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
-    t0 = time.perf_counter()
-    res = oracle.svd(a_host, eps=eps, max_sweeps=sample_sweeps, threads=threads)
-    dt = time.perf_counter() - t0
-    per_sweep = dt / res.sweeps
-    solves_per_s = 1.0 / (per_sweep * gpu_sweeps)
+    host = host_description()
+    threads = min(host["nproc"], int(os.environ.get("OMP_NUM_THREADS", host["nproc"])))
+    times, sweeps = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = oracle.svd(a_host, eps=eps, max_sweeps=max_sweeps, threads=threads)
+        times.append(time.perf_counter() - t0)
+        sweeps = res.sweeps
+    med = float(np.median(times))
     m, n = a_host.shape
+    from jsvd import workloads
+    c1 = workloads.plain_gaussian()
+    oracle.svd(c1, eps=1e-15, max_sweeps=30, threads=threads)   # warm-up
+    c1_times = []
+    for _ in range(c1_reps):
+        t0 = time.perf_counter()
+        c1_res = oracle.svd(c1, eps=1e-15, max_sweeps=30, threads=threads)
+        c1_times.append(time.perf_counter() - t0)
+    svj_times = []
+    try:
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(1)
+    except ImportError:
+        limit = None
+    try:
+        for _ in range(c1_reps):
+            t0 = time.perf_counter()
+            oracle.dgesvj(c1)
+            svj_times.append(time.perf_counter() - t0)
+    finally:
+        if limit is not None:
+            limit.restore_original_limits()
+    c1_med = float(np.median(c1_times))
+    svj_med = float(np.median(svj_times))
     return {
-        "value": solves_per_s,
+        "value": 1.0 / med,
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle C restatement, OpenMP {threads} threads, {res.sweeps} sweeps of the {m}x{n} workload "
-                  f"in {dt:.2f}s ({per_sweep:.3f}s/sweep), scaled to the {gpu_sweeps} sweeps a full solve takes",
+        "sample": f"oracle C restatement of the reference algorithm, OpenMP {threads} threads, median of {reps} full "
+                  f"solves of the {m}x{n} workload ({sweeps} sweeps, {med:.2f} s each; runs "
+                  f"{', '.join(f'{t:.2f}' for t in times)} s)",
+        "host": host,
+        "reference_case_512x256": {
+            "oracle_solves_per_s": 1.0 / c1_med, "oracle_median_s": c1_med, "oracle_sweeps": c1_res.sweeps,
+            "oracle_runs_s": c1_times, "threads": threads,
+            "lapack_dgesvj_solves_per_s": 1.0 / svj_med, "lapack_dgesvj_median_s": svj_med,
+            "lapack_dgesvj_threads": 1 if limit is not None else "library default",
+        },
     }
 
 
@@ -355,6 +432,8 @@ def time_path(torch, jsvd, path, a_host, sigma_true, cfg, steps, warmup, block_w
         s_sorted = np.sort(sig)[::-1]
         acc["sigma_err_normwise"] = float(np.max(np.abs(s_sorted - sigma_true)) / sigma_true[0])
         acc["sigma_relerr_max"] = float(np.max(np.abs(s_sorted - sigma_true) / sigma_true))
+        top = sigma_true >= 1e-3 * sigma_true[0]
+        acc["sigma_relerr_max_top3decades"] = float(np.max(np.abs(s_sorted[top] - sigma_true[top]) / sigma_true[top]))
     lat_ms = np.array(lat) * 1e3
     return {
         "path": jsvd.PATH_NAMES[path],
@@ -377,7 +456,7 @@ def main() -> None:
     ap.add_argument("--workload", default="tall", choices=["tall", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
-    ap.add_argument("--cpu-sample-sweeps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="full CPU-baseline solves of the workload (median)")
     ap.add_argument("--no-file", action="store_true", help="skip the streamed *.mat file leg")
     ap.add_argument("--block-width", type=int, default=0, choices=[0, 16, 32, 64],
                     help="block width of the mixed path (0: the autotuner's choice)")
@@ -441,17 +520,20 @@ def main() -> None:
     reference_case = None
     if args.workload == "tall" and not args.no_plain:
         # the reference's own CPU-runnable case, 512 x 256 N(0,1), eps 1e-15, 30 sweeps, on both paths; sigma
-        # against LAPACK's (numpy.linalg.svd, well conditioned here) and between the paths
+        # against LAPACK's gesdd (numpy.linalg.svd) and its one-sided Jacobi dgejsv, and between the paths
         log("[bench] reference case 512x256")
         c1, _, c1_eps, c1_sweeps, c1_name = make_workload("plain")
         c1_cfg = jsvd.JacobiSvdConfig(epsilon=c1_eps, max_sweeps=c1_sweeps)
         c1_plain = time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, c1, None, c1_cfg, 10, 1)
         c1_mixed = time_path(torch, jsvd, jsvd.PATH_MIXED, c1, None, c1_cfg, 10, 1)
         s_ref = np.sort(np.linalg.svd(c1, compute_uv=False))[::-1]
+        s_jsv = lapack_dgejsv_sigma(c1)
         sig = {}
         for name, rep in (("plain", c1_plain), ("mixed", c1_mixed)):
             sig[name] = np.sort(rep.pop("sigma"))[::-1]
-            rep["accuracy"]["sigma_relerr_vs_lapack"] = float(np.max(np.abs(sig[name] - s_ref) / s_ref))
+            rep["accuracy"]["sigma_relerr_vs_gesdd"] = float(np.max(np.abs(sig[name] - s_ref) / s_ref))
+            if s_jsv is not None:
+                rep["accuracy"]["sigma_relerr_vs_dgejsv"] = float(np.max(np.abs(sig[name] - s_jsv) / s_jsv))
         reference_case = {"workload": c1_name, "plain_f64": c1_plain, "mixed": c1_mixed,
                           "sigma_relerr_mixed_vs_plain": float(np.max(np.abs(sig["mixed"] - sig["plain"]) /
                                                                       sig["plain"])),
@@ -509,7 +591,7 @@ def main() -> None:
     roofline_hbm = round_kernel_hbm_roofline(torch, jsvd)
     torch.cuda.empty_cache()
     log("[bench] device-to-device copy bandwidth (512 MiB, HBM-resident)")
-    copy = device_copy_bandwidth(torch)
+    copy = device_copy_bandwidth(torch, jsvd)
     torch.cuda.empty_cache()
     roofline_hbm["device_copy_gbs"] = copy["achieved_gbs"]
     roofline_hbm["frac_of_device_copy"] = roofline_hbm["achieved"] / copy["achieved_gbs"]
@@ -520,8 +602,7 @@ def main() -> None:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        gpu_plain_sweeps = plain["info"]["sweeps_fp64"] if plain else max_sweeps
-        cpu = cpu_baseline(a_host, eps, gpu_plain_sweeps, args.cpu_sample_sweeps)
+        cpu = cpu_baseline(a_host, eps, max_sweeps, reps=args.cpu_reps)
 
     for rep in (mixed, plain):
         if rep is not None:

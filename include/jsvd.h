@@ -129,8 +129,14 @@ typedef struct jsvd_mixed_config
                                     shape when it measured one, else 32; halved while n <= width */
     int32_t qr_precondition;     /* mixed path: 1 (default) = factor A = Q R in FP64 first and sweep the n x n
                                     R^T (sigma_j = |x_j|, V from the normalised columns X, U = Q V_M); 0 = sweep A */
-    int32_t reserved;
+    int32_t inner_order;         /* block rounds at width 32: JSVD_INNER_CROSS (default) = only the pairs across
+                                    the two blocks (32 inner rounds), plus one step per sweep that sweeps every
+                                    block by itself; JSVD_INNER_FULL = one cyclic sweep over all 64 columns of the
+                                    block pair (63 inner rounds).  Other widths always use the full sweep. */
 } jsvd_mixed_config;
+
+#define JSVD_INNER_CROSS 0
+#define JSVD_INNER_FULL 1
 
 /* Per-solve report of a plan. */
 typedef struct jsvd_solve_info
@@ -545,6 +551,14 @@ int jsvd_offnorm_f64(const double *a, int64_t lda, int64_t m, int64_t n, double 
  */
 int jsvd_u_sigma_f64(const double *a, int64_t lda, int64_t m, int64_t n, double epsilon, double *u,
                      double *sigma, void *stream);
+
+/*
+ * Device-to-device copy of `bytes` (a multiple of 16; both pointers 16-byte aligned) by the library's own
+ * streaming kernel, enqueued on `stream`: the measured practical HBM roof for the bandwidth-bound kernels
+ * (read + write of equal byte counts).  variant 0 = plain stores, 1 = non-temporal stores.  No reference
+ * counterpart (the reference quotes a cost model, tex/sections/03-gpu-parallel-reduction.tex:72-100).
+ */
+int jsvd_hbm_copy(void *dst, const void *src, int64_t bytes, int32_t variant, void *stream);
 
 #ifdef __cplusplus
 }

@@ -593,6 +593,10 @@ namespace
         {
             jsvd::throw_invalid("block_width must be 0, 16, 32 or 64.");
         }
+        if (mx.inner_order != JSVD_INNER_CROSS && mx.inner_order != JSVD_INNER_FULL)
+        {
+            jsvd::throw_invalid("inner_order must be JSVD_INNER_CROSS or JSVD_INNER_FULL.");
+        }
         if (mx.path == JSVD_PATH_MIXED && (mx.fp32_max_sweeps <= 0 || mx.orthogonalize_steps < 1 ||
                                            mx.orthogonalize_steps > 8 || !(mx.fp32_tolerance < 1.0)))
         {

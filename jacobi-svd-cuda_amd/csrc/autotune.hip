@@ -143,14 +143,14 @@ namespace jsvd
         }
 
         // Microseconds per FP32 block round of g on seeded data: up to 8 rounds of a first sweep, best of
-        // `repetitions`.
-        double time_block_rounds(const BlockGeometry &g, int repetitions, hipStream_t stream)
+        // `repetitions`; with the cross ordering also the intra-block step of a sweep (intra_us).
+        double time_block_rounds(const BlockGeometry &g, int repetitions, hipStream_t stream, double &intra_us)
         {
             const size_t a_elems = static_cast<size_t>(g.lda) * g.n_pad;
             const size_t v_elems = static_cast<size_t>(g.ldv) * g.n_pad;
             DevBuf src(sizeof(double) * a_elems), a32(sizeof(float) * a_elems), v32(sizeof(float) * v_elems);
             DevBuf part(sizeof(float) * g.partial_elems()), q(sizeof(float) * g.q_elems());
-            DevBuf flags(sizeof(int) * g.pairs()), perm(sizeof(int) * g.n_pad);
+            DevBuf flags(sizeof(int) * g.flag_elems()), perm(sizeof(int) * g.n_pad);
             hipLaunchKernelGGL(fill_kernel, dim3(grid_for(a_elems)), dim3(256), 0, stream, static_cast<double *>(src.p),
                                static_cast<int64_t>(a_elems), uint64_t{777});
             JSVD_HIP_CHECK(hipGetLastError());
@@ -177,6 +177,23 @@ namespace jsvd
                 JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
                 const double t = ev.elapsed_us() / rounds;
                 us = (us < 0.0) ? t : std::min(us, t);
+            }
+            intra_us = 0.0;
+            if (g.cross)
+            {
+                for (int rep = 0; rep < repetitions; ++rep)
+                {
+                    Events ev;
+                    JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
+                    launch_block_intra<float>(g, static_cast<float *>(a32.p), static_cast<float *>(v32.p), 1e-7,
+                                              static_cast<float *>(part.p), static_cast<float *>(q.p),
+                                              static_cast<int *>(flags.p), nullptr, nullptr,
+                                              static_cast<const int *>(perm.p), nullptr, nullptr, stream);
+                    JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
+                    JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
+                    const double t = ev.elapsed_us();
+                    intra_us = (rep == 0) ? t : std::min(intra_us, t);
+                }
             }
             return us;
         }
@@ -215,14 +232,16 @@ namespace jsvd
             double best_sweep = -1.0;
             for (int wi = 0; wi < 3; ++wi)
             {
-                const BlockGeometry g = BlockGeometry::make(m, n, widths[wi]);
+                BlockGeometry g = BlockGeometry::make(m, n, widths[wi]);
                 if (g.nb < 2)
                 {
                     continue;
                 }
-                const double us = time_block_rounds(g, repetitions, stream);
+                g.cross = widths[wi] == 32; // the solvers' default inner ordering at this width
+                double intra_us = 0.0;
+                const double us = time_block_rounds(g, repetitions, stream, intra_us);
                 ++c.tried;
-                c.sweep_us[wi] = us * g.rounds();
+                c.sweep_us[wi] = us * g.rounds() + intra_us;
                 if (best_sweep < 0.0 || c.sweep_us[wi] < best_sweep)
                 {
                     best_sweep = c.sweep_us[wi];
@@ -235,12 +254,14 @@ namespace jsvd
                 return c;
             }
             BlockGeometry g = BlockGeometry::make(m, n, c.block_width);
+            g.cross = c.block_width == 32;
             double best_us = c.round_us_default; // 512 workgroups, measured above
             c.apply_wgs = g.apply_wgs;
             for (int wgs : {256, 1024})
             {
                 g.apply_wgs = wgs;
-                const double us = time_block_rounds(g, repetitions, stream);
+                double intra_us = 0.0;
+                const double us = time_block_rounds(g, repetitions, stream, intra_us);
                 ++c.tried;
                 if (us < best_us)
                 {

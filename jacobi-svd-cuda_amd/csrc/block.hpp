@@ -26,6 +26,10 @@ namespace jsvd
         int slabs_a = 0;    // lda / kH
         int slabs_v = 0;    // ldv / kH
         int apply_wgs = 512; // target workgroups of one apply launch (two per CU); the autotuner may change it
+        // Inner ordering of a block round at kb = 32: true = only the kb x kb cross pairs of the block pair (kb
+        // inner rounds), with one intra-block step per sweep (launch_block_intra) for the pairs inside each block;
+        // false = one full cyclic sweep over the block pair's 2 kb columns (2 kb - 1 inner rounds).
+        bool cross = false;
 
         static BlockGeometry make(int64_t m, int64_t n, int kb = kB)
         {
@@ -52,6 +56,8 @@ namespace jsvd
         }
         // Elements of the rotation buffer: one kw x kw matrix per pair.
         [[nodiscard]] size_t q_elems() const { return static_cast<size_t>(pairs()) * kw() * kw(); }
+        // Entries of the per-pair apply flags: a pair per outer-round slot, or a block per intra-step slot.
+        [[nodiscard]] size_t flag_elems() const { return static_cast<size_t>(Nb); }
     };
 
     // Block width for an n-column problem: `requested` (0 = the default kB), halved while it leaves fewer than
@@ -74,6 +80,14 @@ namespace jsvd
     void launch_block_round(const BlockGeometry &g, T *a, T *v, int round, double eps, T *partial, T *q,
                             int *apply_flag, int *rot_count, unsigned long long *offmax, const int *perm,
                             double *norm2, const int *skip_if_zero, hipStream_t stream);
+
+    // The intra-block step of a sweep in the cross ordering (g.cross, kb = 32): every block swept by itself (one
+    // cyclic sweep over its kb columns, all blocks at once), with the buffers of launch_block_round (partial and
+    // q hold the intra step's smaller matrices; apply_flag needs g.flag_elems() entries).  No-op when !g.cross.
+    template <typename T>
+    void launch_block_intra(const BlockGeometry &g, T *a, T *v, double eps, T *partial, T *q, int *apply_flag,
+                            int *rot_count, unsigned long long *offmax, const int *perm, double *norm2,
+                            const int *skip_if_zero, hipStream_t stream);
 
     // The block round's inner step alone, on `pairs` independent kw x kw Gram matrices (kw = 2 kb; gram[p]
     // row-major, upper triangle read): one cyclic Jacobi sweep in Gram space; q_out[p] (row-major) receives the

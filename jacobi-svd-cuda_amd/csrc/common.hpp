@@ -89,6 +89,47 @@ namespace jsvd
 
 namespace jsvd::dev
 {
+#ifndef JSVD_SIMT_MODEL
+    // N-element register vector of T (MFMA operands and accumulators, 16-byte loads).
+    template <typename T, int N>
+    using vec_t = T __attribute__((ext_vector_type(N)));
+
+    // Keeps v as computed at this point: an empty asm the compiler cannot see through, so a cross-lane move is
+    // not sunk into a later branch (where it would run with lanes inactive).
+    template <typename T>
+    __device__ inline void keep(T &v)
+    {
+        asm volatile("" : "+v"(v));
+    }
+#else
+    // The CPU model of a workgroup (tests/simt_model) builds the kernels with g++: GCC vector types, and no
+    // register constraint to keep.
+    template <typename T, int N>
+    struct vec_of;
+    template <>
+    struct vec_of<float, 4>
+    {
+        typedef float type __attribute__((vector_size(16)));
+    };
+    template <>
+    struct vec_of<double, 4>
+    {
+        typedef double type __attribute__((vector_size(32)));
+    };
+    template <>
+    struct vec_of<double, 2>
+    {
+        typedef double type __attribute__((vector_size(16)));
+    };
+    template <typename T, int N>
+    using vec_t = typename vec_of<T, N>::type;
+
+    template <typename T>
+    inline void keep(T &)
+    {
+    }
+#endif
+
     // Position `slot` of the round-robin circle at round `r` for an even circle of size N.
     // The circle starts as 0..N-1 (value N-1 is the dummy when the column count is odd) and after every
     // round positions 1..N-1 rotate right by one, so position i >= 1 holds 1 + ((i-1-r) mod (N-1)).

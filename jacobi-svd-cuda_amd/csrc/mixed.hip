@@ -156,6 +156,7 @@ namespace jsvd
         g_ = sweep;
         ga_ = BlockGeometry::make(m, n, sweep.kb);
         ga_.apply_wgs = sweep.apply_wgs;
+        g_.cross = ga_.cross = g_.kb == 32 && mixed_.inner_order == JSVD_INNER_CROSS;
     }
 
     BlockSolver::BlockSolver(int64_t m, int64_t n, const jsvd_config &cfg, const jsvd_mixed_config &mixed,
@@ -184,7 +185,7 @@ namespace jsvd
         JSVD_HIP_CHECK(hipMalloc(&v64_, sizeof(double) * v_elems));
         JSVD_HIP_CHECK(hipMalloc(&partial_, sizeof(double) * part_elems));
         JSVD_HIP_CHECK(hipMalloc(&q_, sizeof(double) * ga_.q_elems()));
-        JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * ga_.pairs()));
+        JSVD_HIP_CHECK(hipMalloc(&flags_, sizeof(int) * ga_.flag_elems()));
         JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * ga_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * ga_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&rot_, sizeof(int) * 2 * cap_));
@@ -269,6 +270,8 @@ namespace jsvd
             {
                 launch_column_order(norm2_, g.n_pad, perm_, skip, stream_);
             }
+            // cross ordering: the pairs inside every block first, then the outer rounds' cross pairs
+            launch_block_intra<T>(g, a, v, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip, stream_);
             for (int r = 0; r < g.rounds(); ++r)
             {
                 launch_block_round<T>(g, a, v, r, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip,

@@ -66,8 +66,11 @@ class MixedConfig(ctypes.Structure):
         ("orthogonalize_steps", ctypes.c_int32),
         ("block_width", ctypes.c_int32),
         ("qr_precondition", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("inner_order", ctypes.c_int32),
     ]
+
+
+INNER_CROSS, INNER_FULL = 0, 1
 
 
 class SolveInfo(ctypes.Structure):
@@ -87,10 +90,13 @@ class SolveInfo(ctypes.Structure):
 
 def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
                  fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None,
-                 block_width: Optional[int] = None, qr_precondition: Optional[bool] = None) -> MixedConfig:
+                 block_width: Optional[int] = None, qr_precondition: Optional[bool] = None,
+                 inner_order: Optional[int] = None) -> MixedConfig:
     mx = MixedConfig()
     lib().jsvd_mixed_config_default(ctypes.byref(mx))
     mx.path = path
+    if inner_order is not None:
+        mx.inner_order = inner_order
     if qr_precondition is not None:
         mx.qr_precondition = int(bool(qr_precondition))
     if block_width is not None:
@@ -191,6 +197,7 @@ def lib() -> ctypes.CDLL:
                                      _vp, _vp, ctypes.POINTER(SolveInfo)]
         L.jsvd_offnorm_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _vp]
         L.jsvd_u_sigma_f64.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _vp, _vp]
+        L.jsvd_hbm_copy.argtypes = [_vp, _vp, _i64, ctypes.c_int32, _vp]
         L.jsvd_block_eigen.argtypes = [ctypes.c_int32, ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_cholesky_f64.argtypes = [_vp, _i64, _i64, _vp, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
@@ -417,6 +424,15 @@ def offnorm(a_col, lda: int, m: int, n: int, out, stream=None) -> None:
 
 def u_sigma(a_col, lda: int, m: int, n: int, eps: float, u, sigma, stream=None) -> None:
     check(lib().jsvd_u_sigma_f64(_ptr(a_col), lda, m, n, eps, _ptr(u), _ptr(sigma), _stream_handle(stream)))
+
+
+def hbm_copy(dst, src, variant: int = 0, stream=None) -> None:
+    """jsvd_hbm_copy: the library's streaming device-to-device copy of src into dst (equal byte sizes, a multiple of
+    16 bytes); variant 1 = non-temporal stores."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("dst and src must have the same byte size")
+    check(lib().jsvd_hbm_copy(_ptr(dst), _ptr(src), nbytes, int(variant), _stream_handle(stream)))
 
 
 def cholesky(a, n: int, info, stream=None) -> None:

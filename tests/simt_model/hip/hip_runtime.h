@@ -10,6 +10,8 @@
 
 #include "../simt.hpp"
 
+#define JSVD_SIMT_MODEL 1
+
 #define __global__
 #define __device__
 #define __host__
@@ -189,6 +191,30 @@ inline int simt_src_down(int delta, int width)
 #define __shfl(v, src, ...) simt_shfl((v), simt_src_idx((src), simt_width(__VA_ARGS__ + 0)), __FILE__, __LINE__)
 #define __shfl_xor(v, mask, ...) simt_shfl((v), simt_src_xor((mask), simt_width(__VA_ARGS__ + 0)), __FILE__, __LINE__)
 #define __shfl_down(v, d, ...) simt_shfl((v), simt_src_down((d), simt_width(__VA_ARGS__ + 0)), __FILE__, __LINE__)
+
+#define __ballot(pred) ::simt::wave_ballot((pred), __FILE__, __LINE__)
+
+typedef float simt_f32x4 __attribute__((vector_size(16)));
+typedef double simt_f64x4 __attribute__((vector_size(32)));
+inline simt_f32x4 simt_mfma_f32(float a, float b, simt_f32x4 c, const char *file, int line)
+{
+    const double ci[4] = {c[0], c[1], c[2], c[3]};
+    double d[4];
+    simt::wave_mfma(simt::WaveOp::mfma_f32, a, b, ci, d, file, line);
+    return simt_f32x4{static_cast<float>(d[0]), static_cast<float>(d[1]), static_cast<float>(d[2]),
+                      static_cast<float>(d[3])};
+}
+inline simt_f64x4 simt_mfma_f64(double a, double b, simt_f64x4 c, const char *file, int line)
+{
+    const double ci[4] = {c[0], c[1], c[2], c[3]};
+    double d[4];
+    simt::wave_mfma(simt::WaveOp::mfma_f64, a, b, ci, d, file, line);
+    return simt_f64x4{d[0], d[1], d[2], d[3]};
+}
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, cbsz, abid, blgp) simt_mfma_f32((a), (b), (c), __FILE__, __LINE__)
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, cbsz, abid, blgp) simt_mfma_f64((a), (b), (c), __FILE__, __LINE__)
+using std::max;
+using std::min;
 
 struct double2
 {

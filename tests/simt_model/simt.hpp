@@ -67,6 +67,9 @@ namespace simt
         dpp,
         readlane,
         shfl,
+        ballot,   // 64-bit mask of the lanes that deposited a non-zero value
+        mfma_f32, // 16x16x4 FP32 matrix multiply-accumulate (operands and accumulators in vin, result in vout)
+        mfma_f64, // 16x16x4 FP64
     };
 
     struct Thread
@@ -83,6 +86,9 @@ namespace simt
         uint32_t old = 0;
         int arg0 = 0, arg1 = 0, arg2 = 0, arg3 = 0;
         uint32_t out = 0;
+        uint64_t out64 = 0;
+        double vin[6] = {};
+        double vout[4] = {};
     };
 
     struct Dim
@@ -329,6 +335,40 @@ namespace simt
         static void compute_wave(Thread *lanes)
         {
             const WaveOp op = lanes[0].op;
+            if (op == WaveOp::ballot)
+            {
+                uint64_t mask = 0;
+                for (int l = 0; l < kWave; ++l)
+                {
+                    mask |= (lanes[l].in != 0) ? (uint64_t(1) << l) : 0;
+                }
+                for (int l = 0; l < kWave; ++l)
+                {
+                    lanes[l].out64 = mask;
+                }
+                return;
+            }
+            if (op == WaveOp::mfma_f32 || op == WaveOp::mfma_f64)
+            {
+                // lane l supplies A[l % 16][l / 16] (vin[0]) and B[l / 16][l % 16] (vin[1]) and holds the accumulator
+                // entries D[row(l, v)][l % 16], v = 0..3 (vin[2..5] in, vout out): row = 4 (l / 16) + v for FP32,
+                // l / 16 + 4 v for FP64.
+                for (int l = 0; l < kWave; ++l)
+                {
+                    const int j = l % 16;
+                    for (int v = 0; v < 4; ++v)
+                    {
+                        const int i = (op == WaveOp::mfma_f32) ? 4 * (l / 16) + v : l / 16 + 4 * v;
+                        double acc = lanes[l].vin[2 + v];
+                        for (int k = 0; k < 4; ++k)
+                        {
+                            acc += lanes[i + 16 * k].vin[0] * lanes[j + 16 * k].vin[1];
+                        }
+                        lanes[l].vout[v] = (op == WaveOp::mfma_f32) ? static_cast<double>(static_cast<float>(acc)) : acc;
+                    }
+                }
+                return;
+            }
             for (int l = 0; l < kWave; ++l)
             {
                 Thread &d = lanes[l];
@@ -453,6 +493,44 @@ namespace simt
         current_group()->park();
     }
 
+    // Matrix multiply-accumulate of a wavefront: returns this lane's four accumulator entries.
+    inline void wave_mfma(WaveOp op, double a, double b, const double (&c)[4], double (&d)[4], const char *file, int line)
+    {
+        Thread *th = current_thread();
+        th->state = Wait::wave;
+        th->op = op;
+        th->in = 0;
+        th->old = 0;
+        th->arg0 = th->arg1 = th->arg2 = th->arg3 = 0;
+        th->vin[0] = a;
+        th->vin[1] = b;
+        for (int v = 0; v < 4; ++v)
+        {
+            th->vin[2 + v] = c[v];
+        }
+        th->file = file;
+        th->line = line;
+        current_group()->park();
+        for (int v = 0; v < 4; ++v)
+        {
+            d[v] = th->vout[v];
+        }
+    }
+
+    inline uint64_t wave_ballot(bool pred, const char *file, int line)
+    {
+        Thread *th = current_thread();
+        th->state = Wait::wave;
+        th->op = WaveOp::ballot;
+        th->in = pred ? 1u : 0u;
+        th->old = 0;
+        th->arg0 = th->arg1 = th->arg2 = th->arg3 = 0;
+        th->file = file;
+        th->line = line;
+        current_group()->park();
+        return th->out64;
+    }
+
     inline uint32_t wave_op(WaveOp op, uint32_t in, uint32_t old, int a0, int a1, int a2, int a3, const char *file,
                             int line)
     {
@@ -478,6 +556,25 @@ namespace simt
         uint64_t barriers = 0;
         uint64_t wave_ops = 0;
     };
+
+    // Two-dimensional grid: workgroups (x, y) for y in 0..grid_y-1, x in 0..grid_x-1, one after another.
+    inline LaunchStats launch2d(int grid_x, int grid_y, int threads, const std::function<void()> &kernel,
+                                size_t stack_bytes = 128 * 1024)
+    {
+        LaunchStats st;
+        for (int y = 0; y < grid_y; ++y)
+        {
+            for (int x = 0; x < grid_x; ++x)
+            {
+                Workgroup wg(threads, Dim{static_cast<unsigned>(x), static_cast<unsigned>(y), 0},
+                             Dim{static_cast<unsigned>(grid_x), static_cast<unsigned>(grid_y), 1}, kernel, stack_bytes);
+                wg.run();
+                st.barriers += wg.barriers();
+                st.wave_ops += wg.wave_ops();
+            }
+        }
+        return st;
+    }
 
     inline LaunchStats launch(int grid, int threads, const std::function<void()> &kernel,
                               std::vector<int> order = {}, size_t stack_bytes = 128 * 1024)

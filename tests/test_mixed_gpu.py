@@ -272,3 +272,32 @@ def test_mixed_rank_deficient_and_zero_inputs(torch_cuda):
     assert rho_orth(got.v) < 1e-11
     assert rho_orth(got.u[:, big]) < 1e-11
     assert rho_a(b, got.u, got.sigma, got.v) < 1e-13
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("shape", [(700, 224), (300, 200)])
+def test_inner_orders_agree(torch_cuda, path, shape):
+    """Block rounds at width 32 in the cross ordering (intra-block step + cross pairs, the default) and in the full
+    ordering give the same sorted sigma (1e-12 relative) and both meet the path's accuracy bounds; 200 columns
+    leave an odd block count (a dummy block in the circle) and a padded last block."""
+    m, n = shape
+    a = gaussian(m, n, 17) * 10.0 ** (-4 * np.arange(n) / n)
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    out = {}
+    for order in (jsvd.INNER_CROSS, jsvd.INNER_FULL):
+        got, info = jsvd.solve(a, cfg, path=path, mixed=jsvd.mixed_config(path, block_width=32, inner_order=order))
+        assert info.converged == 1, order
+        assert rho_a(a, got.u, got.sigma, got.v) < 1e-13, order
+        assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11, order
+        out[order] = sorted_desc(got.sigma)
+    s_ref = sorted_desc(np.linalg.svd(a, compute_uv=False))
+    for order, s in out.items():
+        assert np.max(np.abs(s - s_ref) / s_ref) < 1e-12, order
+    assert np.max(np.abs(out[jsvd.INNER_CROSS] - out[jsvd.INNER_FULL]) / out[jsvd.INNER_FULL]) < 1e-12
+
+
+def test_inner_order_validated(torch_cuda):
+    a = gaussian(80, 64, 1)
+    with pytest.raises(ValueError, match="inner_order"):
+        jsvd.solve(a, jsvd.JacobiSvdConfig(epsilon=1e-14), path=jsvd.PATH_MIXED,
+                   mixed=jsvd.mixed_config(jsvd.PATH_MIXED, inner_order=7))

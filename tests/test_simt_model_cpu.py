@@ -1,11 +1,16 @@
-"""The batched kernel on the CPU model of a workgroup (tests/simt_model), no GPU needed.
+"""Kernels of the library on the CPU model of a workgroup (tests/simt_model), no GPU needed.
 
-tests/simt_model compiles the library's kernel source (csrc/batch_kernel.hpp, unchanged) with g++ and
-AddressSanitizer and runs every GPU thread as a fiber: a barrier or wavefront operation that the whole
-workgroup / wavefront does not reach, or any access outside LDS, scratch slices or the packed buffers, stops
-the run with an error.  The inputs are those of the batch test held off the GPU (1x1, 5x1, 2x2, 64x64, the
-at-limit 256x128 and 255x127 items, and a batch with an item beyond the limits), under every launch shape of
-the kernel.  The model's own checks are pinned by tests/simt_model/selftest.cpp (planted faults).
+tests/simt_model compiles the library's kernel sources (csrc/batch_kernel.hpp, csrc/block_kernels.hpp,
+unchanged) with g++, AddressSanitizer and UBSan and runs every GPU thread as a fiber: a barrier or wavefront
+operation (DPP move, ballot, MFMA) that the whole workgroup / wavefront does not reach, or any access outside LDS,
+scratch slices or the buffers, stops the run with an error.
+
+* Batched kernel: the inputs of the batch test held off the GPU (1x1, 5x1, 2x2, 64x64, the at-limit 256x128 and
+  255x127 items, and a batch with an item beyond the limits), under every launch shape of the kernel.
+* Block kernels: the sweep loop of the block and mixed paths (column ordering, intra-block step, Gram / inner
+  sweep / apply rounds) at width 32 in both inner orderings, FP64 and FP32, against LAPACK's singular values.
+
+The model's own checks are pinned by tests/simt_model/selftest.cpp (planted faults).
 """
 import concurrent.futures as cf
 import os
@@ -36,9 +41,9 @@ def _build(name):
 
 @pytest.fixture(scope="module")
 def model_bins():
-    with cf.ThreadPoolExecutor(2) as ex:
-        batch, selftest = ex.map(_build, ["batch_model", "selftest"])
-    return batch, selftest
+    with cf.ThreadPoolExecutor(3) as ex:
+        batch, selftest, block = ex.map(_build, ["batch_model", "selftest", "block_model"])
+    return batch, selftest, block
 
 
 def _run_model(binary, mats, prec, tol, threads, lds, max_sweeps, max_rows=256, max_cols=128):
@@ -76,7 +81,7 @@ def _edge_inputs():
 
 
 def test_model_catches_planted_faults(model_bins):
-    _, selftest = model_bins
+    _, selftest, _ = model_bins
     r = subprocess.run([selftest, "sum"], capture_output=True, text=True, env=ENV)
     assert r.returncode == 0 and r.stdout.split() == ["2016.0"] * 4, r.stdout + r.stderr
     for case, needles in (("barrier", ["left the kernel"]), ("divergent", ["lanes at a wave op"]),
@@ -88,7 +93,7 @@ def test_model_catches_planted_faults(model_bins):
 
 def test_batch_edge_items_on_model_every_shape(model_bins):
     """1x1, 5x1, 2x2, 64x64 (max 60 sweeps) and the rejection batch, under all six launch shapes."""
-    binary, _ = model_bins
+    binary = model_bins[0]
     mats, big = _edge_inputs()
     small = [mats[0], mats[1], mats[2], mats[5]]
     refs = [oracle.svd(a, eps=1e-14, max_sweeps=60) for a in small]
@@ -114,7 +119,7 @@ def test_batch_at_limit_items_on_model(model_bins):
     """256x128 and 255x127 (A and V in the scratch slice at the 38/76 KiB shapes, V in LDS at 152 KiB): two
     sweeps cover every index the later sweeps use.  Two sweeps leave sigma unconverged, where the kernel's and the
     oracle's summation orders give slightly different rotations: column norms agree to 1e-9 relative."""
-    binary, _ = model_bins
+    binary = model_bins[0]
     mats, _ = _edge_inputs()
     big = [mats[3], mats[4]]
     refs = [oracle.svd(a, eps=1e-14, max_sweeps=2) for a in big]
@@ -126,3 +131,54 @@ def test_batch_at_limit_items_on_model(model_bins):
             for i, ref in enumerate(refs):
                 assert sweeps[i] == 2, key
                 assert np.max(np.abs(sig[i] - ref.sigma) / ref.sigma) < 1e-9, key
+
+
+def _run_block_model(binary, a, prec, order, kb, max_sweeps, eps):
+    """The block sweep loop on the model; returns (sweeps, rotations per sweep, A (m x n), V (n x n))."""
+    m, n = a.shape
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in"), os.path.join(d, "out")
+        with open(fi, "wb") as f:
+            np.array([m, n], np.int64).tofile(f)
+            np.ascontiguousarray(a.T).tofile(f)  # column-major
+        r = subprocess.run([binary, prec, order, str(kb), str(max_sweeps), repr(eps), fi, fo], capture_output=True,
+                           text=True, env=ENV)
+        assert r.returncode == 0, r.stderr[-4000:]
+        raw = open(fo, "rb").read()
+    sweeps, lda, ldv, npad = (int(x) for x in np.frombuffer(raw[:32], np.int64))
+    o = 48
+    rots = np.frombuffer(raw[o:o + 4 * sweeps], np.int32)
+    o += 12 * sweeps
+    a_out = np.frombuffer(raw[o:o + 8 * lda * npad], np.float64).reshape(npad, lda).T
+    o += 8 * lda * npad
+    v_out = np.frombuffer(raw[o:o + 8 * ldv * npad], np.float64).reshape(npad, ldv).T
+    return sweeps, rots, a_out[:m, :n], v_out[:n, :n]
+
+
+def test_block_sweeps_on_model(model_bins):
+    """Width-32 block sweeps of a column-graded 160 x 128 matrix (4 blocks) to convergence: the cross ordering
+    (intra-block step + cross pairs) visits every column pair once per sweep; both orderings end on a rotation-free
+    sweep with LAPACK's singular values (FP64 1e-13, FP32 1e-5 of sigma_max), V orthogonal and A V = X."""
+    block = model_bins[2]
+    rng = np.random.default_rng(5)
+    n = 128
+    a = rng.standard_normal((n + 32, n)) * 10.0 ** (-3 * np.arange(n) / n)
+    ref = np.linalg.svd(a, compute_uv=False)
+    cases = [("f64", "cross", 1e-14), ("f64", "full", 1e-14), ("f32", "cross", 1e-6)]
+    with cf.ThreadPoolExecutor(3) as ex:
+        futs = [ex.submit(_run_block_model, block, a, p, o, 32, 30, e) for p, o, e in cases]
+        for (prec, order, _), fut in zip(cases, futs):
+            sweeps, rots, x, v = fut.result()
+            assert rots[-1] == 0 and sweeps < 30, (prec, order, list(rots))
+            if order == "cross":
+                # each column pair is visited once per sweep (and in FP64 every pair rotates in the first sweep)
+                assert np.all(rots <= n * (n - 1) // 2), (prec, list(rots))
+                assert prec != "f64" or rots[0] == n * (n - 1) // 2, list(rots)
+            s = np.sort(np.linalg.norm(x, axis=0))[::-1]
+            if prec == "f64":
+                assert np.max(np.abs(s - ref) / ref) < 1e-13, order
+                assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-12, order
+                assert np.linalg.norm(a @ v - x) / np.linalg.norm(a) < 1e-14, order
+            else:
+                assert np.max(np.abs(s - ref)) / ref[0] < 1e-5, order
+                assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-4, order
