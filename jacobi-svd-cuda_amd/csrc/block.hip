@@ -72,14 +72,14 @@ namespace jsvd
                                            rot_count, offmax, perm, norm2, skip_if_zero, stream);
         }
 
-        template <typename T, int KB>
+        template <typename T, int KB, bool CROSS>
         void block_eigen_kb(const T *gram, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
         {
             // A circle of 2 * pairs blocks at round 0 maps slot p to a valid block pair, so the kernel's pair p
             // reads gram[p] (one partial) and writes q_out[p] and rotations[p]; no column map, norms or sweep
             // counters.
             const int Nb = 2 * pairs;
-            hipLaunchKernelGGL((block_eigen_kernel<T, KB, false>), dim3(pairs), dim3(EigenShape<KB>::kThreads), 0,
+            hipLaunchKernelGGL((block_eigen_kernel<T, KB, CROSS>), dim3(pairs), dim3(EigenShape<KB>::kThreads), 0,
                                stream, gram, 1, Nb, Nb, 0, eps, q_out, rotations, nullptr, nullptr, nullptr, nullptr,
                                nullptr);
             JSVD_HIP_CHECK(hipGetLastError());
@@ -142,19 +142,29 @@ namespace jsvd
                                              const int *, hipStream_t);
 
     template <typename T>
-    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations, hipStream_t stream)
+    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations, bool cross,
+                            hipStream_t stream)
     {
+        if (cross)
+        {
+            if (kb != 32)
+            {
+                throw_invalid("the cross ordering runs at block width 32.");
+            }
+            block_eigen_kb<T, 32, true>(gram, pairs, eps, q_out, rotations, stream);
+            return;
+        }
         switch (kb)
         {
-        case 16: block_eigen_kb<T, 16>(gram, pairs, eps, q_out, rotations, stream); break;
-        case 32: block_eigen_kb<T, 32>(gram, pairs, eps, q_out, rotations, stream); break;
-        case 64: block_eigen_kb<T, 64>(gram, pairs, eps, q_out, rotations, stream); break;
+        case 16: block_eigen_kb<T, 16, false>(gram, pairs, eps, q_out, rotations, stream); break;
+        case 32: block_eigen_kb<T, 32, false>(gram, pairs, eps, q_out, rotations, stream); break;
+        case 64: block_eigen_kb<T, 64, false>(gram, pairs, eps, q_out, rotations, stream); break;
         default: throw_invalid("block width must be 16, 32 or 64.");
         }
     }
 
-    template void launch_block_eigen<float>(const float *, int, int, double, float *, int *, hipStream_t);
-    template void launch_block_eigen<double>(const double *, int, int, double, double *, int *, hipStream_t);
+    template void launch_block_eigen<float>(const float *, int, int, double, float *, int *, bool, hipStream_t);
+    template void launch_block_eigen<double>(const double *, int, int, double, double *, int *, bool, hipStream_t);
 
     void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
                              hipStream_t stream)

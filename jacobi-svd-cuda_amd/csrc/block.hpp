@@ -92,8 +92,10 @@ namespace jsvd
     // The block round's inner step alone, on `pairs` independent kw x kw Gram matrices (kw = 2 kb; gram[p]
     // row-major, upper triangle read): one cyclic Jacobi sweep in Gram space; q_out[p] (row-major) receives the
     // accumulated rotation and rotations[p] the number of rotated column pairs.
+    // cross = true: the cross ordering of the block rounds at kb = 32 (only the kb x kb pairs across the two
+    // blocks, kb inner rounds).
     template <typename T>
-    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations,
+    void launch_block_eigen(const T *gram, int kb, int pairs, double eps, T *q_out, int *rotations, bool cross,
                             hipStream_t stream);
 
     // Largest padded column count the one-workgroup ordering kernel sorts (LDS bound).

@@ -346,19 +346,15 @@ namespace jsvd
             return own ? v : moved;
         }
 
-        // J_t^T G J_u of one 2x2 block, or an exact copy of it when neither pair rotates.  Both rotations are
-        // loaded and applied unconditionally and the copy is a select, so the LDS reads of the block and of the
-        // rotations go out together (one LDS round trip on the inner round's dependent chain, not two).
+        // J_t^T G J_u of one 2x2 block with the rotations of its two pairs.  A pair that does not rotate carries
+        // exactly (c, s) = (1, 0), with which rotate_block returns its inputs unchanged (each product with 0 is a
+        // zero and each fused multiply-add by 1 is exact), so no per-pair rotation flag is read: the LDS reads of
+        // the block and of the two rotations are the whole of a G thread's inputs.
         template <typename T>
-        __device__ inline void rotate_or_copy(T g11, T g12, T g21, T g22, const T (&rt)[2], const T (&ru)[2],
-                                              bool rotates, T &n11, T &n12, T &n21, T &n22)
+        __device__ inline void rotate_pair_block(T g11, T g12, T g21, T g22, const T (&rt)[2], const T (&ru)[2], T &n11,
+                                                 T &n12, T &n21, T &n22)
         {
-            T r11, r12, r21, r22;
-            rotate_block(rt[0], rt[1], ru[0], ru[1], g11, g12, g21, g22, r11, r12, r21, r22);
-            n11 = rotates ? r11 : g11;
-            n12 = rotates ? r12 : g12;
-            n21 = rotates ? r21 : g21;
-            n22 = rotates ? r22 : g22;
+            rotate_block(rt[0], rt[1], ru[0], ru[1], g11, g12, g21, g22, n11, n12, n21, n22);
         }
 
         // One inner round on a thread's Q registers: rotate its QP position pairs, then move the positions one
@@ -432,11 +428,12 @@ namespace jsvd
         template <typename T>
         __device__ inline bool block_rotation(double app, double aqq, double apq, double eps, double &c, double &s)
         {
-            // The FP32 phase applies its rotations in FP32, so the shorter FP64 chain is exact enough there; the
-            // FP64 phase keeps the division form.
+            // The FP32 phase applies its rotations in FP32 to FP32 Gram entries, so the short FP64 chain (hardware
+            // reciprocal square roots with one Newton step, no scaling) is exact enough there; the FP64 phase
+            // keeps the division form.
             if constexpr (sizeof(T) == 4)
             {
-                return dev::rotation_params_fast(app, aqq, apq, eps, c, s);
+                return dev::rotation_params_f32(app, aqq, apq, eps, c, s);
             }
             else
             {
@@ -468,7 +465,6 @@ namespace jsvd
             using L = GramLds<T, KB>;
             __shared__ T g[2][L::kElems];               // G of the current round and of the next
             __shared__ T cs[2][kPairs][2];              // rotations of the current round and of the next
-            __shared__ unsigned long long rmask[2];     // bit u: pair u rotates
             __shared__ double red[kThreads / 64];
             __shared__ int any;
             __shared__ int needed;
@@ -615,6 +611,13 @@ namespace jsvd
                 }
             }
             const bool run = needed != 0;
+            // The rotation wave's chain (its reads of G, the FP64 parameters, the write of the next rotations) is
+            // the dependent path of every inner round; the other waves' LDS traffic queues behind its requests
+            // only if it issues first, so it runs at the highest issue priority.
+            if (tid < 64)
+            {
+                __builtin_amdgcn_s_setprio(3);
+            }
             // Wave 0: the rotation of pair pu of the current round and the three G entries it was formed from.
             const int pu = tid % kPairs;
             T my_c = 1, my_s = 0, gpp = 0, gqq = 0, gpq = 0;
@@ -630,16 +633,11 @@ namespace jsvd
                                                    static_cast<double>(gpq), eps, c, s);
                 my_c = static_cast<T>(c);
                 my_s = static_cast<T>(s);
-                const unsigned long long m = __ballot(rot);
                 if (tid < kPairs)
                 {
                     cs[0][tid][0] = my_c;
                     cs[0][tid][1] = my_s;
                     my_rotations += rot ? 1 : 0;
-                }
-                if (tid == 0)
-                {
-                    rmask[0] = m;
                 }
             }
             __syncthreads();
@@ -650,21 +648,20 @@ namespace jsvd
                 T *gn = g[b ^ 1];
                 if (gthr)
                 {
-                    const unsigned long long m = rmask[b];
                     if constexpr (kBPT == 1)
                     {
                         if (btu[0] >= 0)
                         {
                             const int t = btu[0] & 255, u = btu[0] >> 8;
                             const int c1 = x1[0], c2 = x2[0], c3 = z1[0], c4 = z2[0];
-                            // J_t^T G J_u of the block into the other buffer (copied when neither pair rotates)
+                            // J_t^T G J_u of the block into the other buffer
                             const int o11 = L::at(c1, c3);
                             const int o12 = L::at(c1, c4);
                             const int o21 = L::at(c2, c3);
                             const int o22 = L::at(c2, c4);
                             T n11, n12, n21, n22;
-                            rotate_or_copy<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u],
-                                              ((m >> t) | (m >> u)) & 1ull, n11, n12, n21, n22);
+                            rotate_pair_block<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u], n11, n12,
+                                                 n21, n22);
                             gn[o11] = n11;
                             gn[o12] = n12;
                             gn[o22] = n22;
@@ -704,8 +701,8 @@ namespace jsvd
                                 const int o21 = L::at(c2, c3);
                                 const int o22 = L::at(c2, c4);
                                 T n11, n12, n21, n22;
-                                rotate_or_copy<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u],
-                                                  ((m >> t) | (m >> u)) & 1ull, n11, n12, n21, n22);
+                                rotate_pair_block<T>(gc[o11], gc[o12], gc[o21], gc[o22], cs[b][t], cs[b][u], n11,
+                                                     n12, n21, n22);
                                 gn[o11] = n11;
                                 gn[o12] = n12;
                                 gn[o22] = n22;
@@ -759,16 +756,11 @@ namespace jsvd
                                                        static_cast<double>(gpq), eps, c, s);
                     my_c = static_cast<T>(c);
                     my_s = static_cast<T>(s);
-                    const unsigned long long m = __ballot(rot);
                     if (tid < kPairs)
                     {
                         cs[b ^ 1][tid][0] = my_c;
                         cs[b ^ 1][tid][1] = my_s;
                         my_rotations += rot ? 1 : 0;
-                    }
-                    if (tid == 0)
-                    {
-                        rmask[b ^ 1] = m;
                     }
                 }
                 if (!CROSS && tid < 64 && rr + 1 < kRounds)
@@ -811,16 +803,11 @@ namespace jsvd
                                                        static_cast<double>(gpq), eps, c, s);
                     my_c = static_cast<T>(c);
                     my_s = static_cast<T>(s);
-                    const unsigned long long m = __ballot(rot);
                     if (tid < kPairs)
                     {
                         cs[b ^ 1][tid][0] = my_c;
                         cs[b ^ 1][tid][1] = my_s;
                         my_rotations += rot ? 1 : 0;
-                    }
-                    if (tid == 0)
-                    {
-                        rmask[b ^ 1] = m;
                     }
                 }
                 __syncthreads();

@@ -229,6 +229,38 @@ namespace jsvd::dev
         return rot;
     }
 
+    // 1 / sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, about 2^-23 relative) refined by one Newton step
+    // y (3 - x y^2) / 2, which leaves about 2^-45 relative.
+    __device__ inline double rsqrt_newton(double x)
+    {
+        const double y = __builtin_amdgcn_rsq(x);
+        const double e = fma(-x * y, y, 1.0); // 1 - x y^2
+        return fma(0.5 * y, e, y);
+    }
+
+    // Rotation parameters for Gram entries that are FP32 values (the FP32 phase of the block paths, whose
+    // rotations are applied in FP32): the skip test of rotation_params on squares (apq^2 > eps^2 app aqq, no square
+    // root), then the same rotation as rotation_params_block (t = sign(d) g / (|d| + hypot(d, g)),
+    // c = 1 / sqrt(1 + t^2)) as c = den w, s = sign(d) g w with den = |d| + hypot(d, g), w = 1 / hypot(den, g), the
+    // two square roots by the hardware estimate and one Newton step (about 2^-45 relative, far below FP32
+    // rounding).  FP32 inputs have no overflow or underflow in these FP64 squares, so no scaling is needed.
+    // Branch-free: a skipped pair gets (1, 0).
+    __device__ inline bool rotation_params_f32(double app, double aqq, double apq, double eps, double &c, double &s)
+    {
+        const double pq = fmax(app, 0.0) * fmax(aqq, 0.0);
+        const bool rot = (apq * apq > (eps * eps) * pq) && fabs(apq) > 1.0e-300;
+        const double d = aqq - app;
+        const double g = 2.0 * apq;
+        const double g2 = g * g;
+        const double h2 = fma(d, d, g2);
+        const double hyp = h2 * rsqrt_newton(h2);
+        const double den = fabs(d) + hyp;
+        const double w = rsqrt_newton(fma(den, den, g2));
+        c = rot ? den * w : 1.0;
+        s = rot ? ((d >= 0.0) ? g : -g) * w : 0.0;
+        return rot;
+    }
+
     // Off-orthogonality of a pair, |gamma| / sqrt(alpha beta), 0 when a column is zero.
     __host__ __device__ inline double pair_cosine(double app, double aqq, double apq)
     {

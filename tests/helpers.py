@@ -47,17 +47,23 @@ def block_rotation(app, aqq, apq, eps):
     return c, t * c, True
 
 
-def gram_space_sweep(g, eps):
+def gram_space_sweep(g, eps, cross=False):
     """One cyclic Jacobi sweep on the symmetric n x n Gram matrix g in round-robin order (pair u of a round =
-    circle positions u and n-1-u): g <- J^T g J per rotation, Q accumulates J.  Returns (Q, rotations, g)."""
+    circle positions u and n-1-u), or with cross=True only over the (n/2)^2 pairs across the two halves (round r
+    pairs column u with column n/2 + (u + r) mod n/2, n/2 rounds): g <- J^T g J per rotation, Q accumulates J.
+    Returns (Q, rotations, g)."""
     g = np.array(g, dtype=np.float64)
     n = g.shape[0]
+    h = n // 2
     q = np.eye(n)
     rotations = 0
-    for rnd in range(n - 1):
+    for rnd in range(h if cross else n - 1):
         params = []
         for u in range(n // 2):
-            p, r = circle_at(u, rnd, n), circle_at(n - 1 - u, rnd, n)
+            if cross:
+                p, r = u, h + (u + rnd) % h
+            else:
+                p, r = circle_at(u, rnd, n), circle_at(n - 1 - u, rnd, n)
             c, s, rot = block_rotation(g[p, p], g[r, r], g[p, r], eps)
             rotations += rot
             params.append((p, r, c, s))

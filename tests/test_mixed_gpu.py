@@ -148,6 +148,44 @@ def test_block_eigen_matches_gram_space_sweep(torch_cuda, dtype, tol, kw):
         assert np.max(np.abs(swept - g_ref)) / np.max(np.abs(g_in)) < (1e-13 if dtype == "float64" else 1e-5)
 
 
+@pytest.mark.parametrize("dtype,tol", [("float64", 1e-12), ("float32", 2e-4)])
+def test_block_eigen_cross_matches_gram_space_sweep(torch_cuda, dtype, tol):
+    """The cross ordering of the block rounds (jsvd_block_eigen_ordered, kw = 64) against the numpy restatement of
+    the same ordering (tests/helpers.py, cross=True): Q agrees to rounding, rotation counts agree (FP64), Q is
+    orthogonal and Q^T G Q equals the swept G."""
+    import torch
+    from helpers import gram_space_sweep
+    rng = np.random.default_rng(77)
+    pairs, kw = 5, 64
+    grams = []
+    for p in range(pairs):
+        x = rng.standard_normal((300, kw)) * 10.0 ** (-(p + 1) * np.arange(kw) / (kw - 1.0))
+        x[:, kw - 1 - p] = 0.0
+        grams.append(x.T @ x)
+    tdt = getattr(torch, dtype)
+    d_g = torch.from_numpy(np.stack(grams)).to(device="cuda", dtype=tdt).contiguous()
+    d_q = torch.empty_like(d_g)
+    d_rot = torch.zeros(pairs, dtype=torch.int32, device="cuda")
+    eps = 1e-15 if dtype == "float64" else 1e-7
+    jsvd.block_eigen(d_g, eps, d_q, d_rot, inner_order=jsvd.INNER_CROSS)
+    torch.cuda.synchronize()
+    q = d_q.cpu().numpy().astype(np.float64)
+    rot = d_rot.cpu().numpy()
+    for p in range(pairs):
+        g_in = d_g[p].cpu().numpy().astype(np.float64)
+        q_ref, rot_ref, g_ref = gram_space_sweep(g_in, eps, cross=True)
+        assert np.max(np.abs(q[p] - q_ref)) < tol, p
+        if dtype == "float64":
+            assert rot[p] == rot_ref
+        assert rho_orth(q[p]) < (1e-13 if dtype == "float64" else 1e-5)
+        swept = q[p].T @ g_in @ q[p]
+        assert np.max(np.abs(swept - g_ref)) / np.max(np.abs(g_in)) < (1e-13 if dtype == "float64" else 1e-5)
+    with pytest.raises(ValueError, match="inner_order"):
+        g16 = torch.zeros((2, 32, 32), dtype=tdt, device="cuda")
+        jsvd.block_eigen(g16, eps, torch.empty_like(g16), torch.zeros(2, dtype=torch.int32, device="cuda"),
+                         inner_order=jsvd.INNER_CROSS)
+
+
 def test_block_eigen_rejects_bad_arguments(torch_cuda):
     import torch
     g = torch.zeros((2, 64, 64), dtype=torch.float64, device="cuda")

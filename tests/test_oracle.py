@@ -239,3 +239,30 @@ def test_gram_space_sweep_reference():
     c, s, _ = block_rotation(app, aqq, apq, 1e-15)
     assert abs(apq * (c * c - s * s) + (app - aqq) * c * s) < 1e-15
     assert abs(apq * (c * c - s * s) + (app - aqq) * c * (-s)) > 0.1
+
+
+def test_gram_space_cross_sweep_reference():
+    """The cross ordering of the numpy Gram-space sweep: every pair (i, h + j) across the two halves is rotated
+    exactly once per sweep (the first sweep of a full-rank G rotates all h^2 of them), each sweep is an orthogonal
+    similarity, and alternating it with full sweeps of each half (the intra-block step) diagonalises G to its
+    eigenvalues; pairs inside a half are never touched by the cross sweep."""
+    from helpers import gram_space_sweep
+    rng = np.random.default_rng(4)
+    n, h = 16, 8
+    x = rng.standard_normal((120, n)) * 10.0 ** (-2 * np.arange(n) / (n - 1))
+    g = x.T @ x
+    q, rot, g1 = gram_space_sweep(g, 1e-15, cross=True)
+    assert rot == h * h
+    assert np.linalg.norm(q.T @ q - np.eye(n)) < 1e-14
+    assert np.max(np.abs(q.T @ g @ q - g1)) < 1e-14 * np.max(np.abs(g))
+    gi = g1
+    for _ in range(8):
+        for lo in (0, h):
+            qi, _, _ = gram_space_sweep(gi[lo:lo + h, lo:lo + h], 1e-15)
+            full = np.eye(n)
+            full[lo:lo + h, lo:lo + h] = qi
+            gi = full.T @ gi @ full
+        _, rot, gi = gram_space_sweep(gi, 1e-15, cross=True)
+    assert rot == 0
+    ev = np.linalg.eigvalsh(g)
+    assert np.max(np.abs(np.sort(np.diag(gi)) - ev) / ev) < 1e-11
