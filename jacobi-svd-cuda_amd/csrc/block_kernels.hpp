@@ -226,12 +226,12 @@ namespace jsvd
         // row length keeps both address steps that neighbouring lanes take (+1 along a row, row length - 1 across
         // rows) odd, so a wave's accesses spread over all LDS banks (an odd row length makes the second step a
         // multiple of the bank count).
-        template <typename T, int KB>
+        template <typename T, int KB, int STRIDE = 0>
         struct GramLds
         {
             static constexpr int KW = 2 * KB;
             static constexpr bool kCirc = KB == 64 && sizeof(T) == 8;
-            static constexpr int kStride = kCirc ? KW / 2 + 2 : (KB == 64 ? KW + 1 : KW + 2);
+            static constexpr int kStride = STRIDE ? STRIDE : (kCirc ? KW / 2 + 2 : (KB == 64 ? KW + 1 : KW + 2));
             static constexpr int kElems = KW * kStride;
             __device__ static int at(int i, int j)
             {
@@ -462,7 +462,9 @@ namespace jsvd
             // Inner rounds: the full cyclic sweep over the kw columns (circle of kw positions), or the cross
             // ordering (slot u pairs column u of the first block with column kb + (u + r) mod kb of the second).
             constexpr int kRounds = CROSS ? KB : KW - 1;
-            using L = GramLds<T, KB>;
+            // the cross ordering's access pattern: rows of kw + 5 slots (a sweep of kw + 1 .. kw + 9 on the device:
+            // kw + 5, kw + 6 and kw + 9 about 2 % faster than kw + 2, kw + 8 10 % slower)
+            using L = GramLds<T, KB, CROSS ? KW + 5 : 0>;
             __shared__ T g[2][L::kElems];               // G of the current round and of the next
             __shared__ T cs[2][kPairs][2];              // rotations of the current round and of the next
             __shared__ double red[kThreads / 64];
@@ -497,34 +499,74 @@ namespace jsvd
                 btu[k] = (gthr && gi < kTri) ? (t | ((t + rem) << 8)) : -1;
             }
             const T *base = partial + static_cast<int64_t>(slot) * slabs * (KW * KW);
-            for (int e = tid; e < KW * KW; e += kThreads)
+            if constexpr (KW * KW == 4 * kThreads)
             {
-                const int i = e / KW;
-                const int j = e % KW;
-                if (i <= j)
+                // four consecutive entries of one row per thread, as 4-element vector loads (eight slabs in
+                // flight per step); the summation order per entry is the slab order, as below
+                using V4 = dev::vec_t<T, 4>;
+                const int e0 = 4 * tid;
+                V4 acc = V4{0, 0, 0, 0};
+                int s = 0;
+                for (; s + 8 <= slabs; s += 8)
                 {
-                    // eight independent loads in flight per step; the summation order is fixed (slab order)
-                    T acc = 0;
-                    int s = 0;
-                    for (; s + 8 <= slabs; s += 8)
-                    {
-                        T part[8];
+                    V4 part[8];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k)
-                        {
-                            part[k] = base[static_cast<int64_t>(s + k) * (KW * KW) + e];
-                        }
-#pragma unroll
-                        for (int k = 0; k < 8; ++k)
-                        {
-                            acc += part[k];
-                        }
-                    }
-                    for (; s < slabs; ++s)
+                    for (int k = 0; k < 8; ++k)
                     {
-                        acc += base[static_cast<int64_t>(s) * (KW * KW) + e];
+                        part[k] = *reinterpret_cast<const V4 *>(base + static_cast<int64_t>(s + k) * (KW * KW) + e0);
                     }
-                    g[0][L::at(i, j)] = acc;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        acc += part[k];
+                    }
+                }
+                for (; s < slabs; ++s)
+                {
+                    acc += *reinterpret_cast<const V4 *>(base + static_cast<int64_t>(s) * (KW * KW) + e0);
+                }
+                const int i = e0 / KW;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                {
+                    const int j = e0 % KW + k;
+                    if (i <= j)
+                    {
+                        g[0][L::at(i, j)] = acc[k];
+                    }
+                }
+            }
+            else
+            {
+                for (int e = tid; e < KW * KW; e += kThreads)
+                {
+                    const int i = e / KW;
+                    const int j = e % KW;
+                    if (i <= j)
+                    {
+                        // eight independent loads in flight per step; the summation order is fixed (slab order)
+                        T acc = 0;
+                        int s = 0;
+                        for (; s + 8 <= slabs; s += 8)
+                        {
+                            T part[8];
+    #pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                            {
+                                part[k] = base[static_cast<int64_t>(s + k) * (KW * KW) + e];
+                            }
+    #pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                            {
+                                acc += part[k];
+                            }
+                        }
+                        for (; s < slabs; ++s)
+                        {
+                            acc += base[static_cast<int64_t>(s) * (KW * KW) + e];
+                        }
+                        g[0][L::at(i, j)] = acc;
+                    }
                 }
             }
             if (tid == 0)
@@ -555,21 +597,35 @@ namespace jsvd
             // Off-orthogonality of the pair's columns before the sweep (convergence measure), and whether any
             // pair passes the skip test at all: if none does, no rotation can happen in the sweep (G never
             // changes) and the sweep is skipped outright.
+            // Both on squares (no square root per pair): the largest squared cosine gij^2 / (gii gjj), and the skip
+            // test gij^2 > eps^2 gii gjj.  The cross ordering checks only the pairs it rotates (one per thread at
+            // kb = 32); the pairs inside each block are checked by the intra-block step of the same sweep.
             double best = 0.0;
             bool need = false;
-            for (int e = tid; e < KW * KW; e += kThreads)
+            auto scan = [&](int i, int j) {
+                const double gii = fmax(static_cast<double>(g[0][L::at(i, i)]), 0.0);
+                const double gjj = fmax(static_cast<double>(g[0][L::at(j, j)]), 0.0);
+                const double gij = static_cast<double>(g[0][L::at(i, j)]);
+                const double den = gii * gjj;
+                const double g2 = gij * gij;
+                best = (den > 0.0) ? fmax(best, g2 / den) : best;
+                need = need || (g2 > (eps * eps) * den && fabs(gij) > 1.0e-300);
+            };
+            if constexpr (CROSS)
             {
-                const int i = e / KW;
-                const int j = e % KW;
-                if (i < j)
+                static_assert(KB * KB == kThreads, "one cross pair per thread");
+                scan(tid / KB, KB + tid % KB);
+            }
+            else
+            {
+                for (int e = tid; e < KW * KW; e += kThreads)
                 {
-                    const double gii = static_cast<double>(g[0][L::at(i, i)]);
-                    const double gjj = static_cast<double>(g[0][L::at(j, j)]);
-                    const double gij = static_cast<double>(g[0][L::at(i, j)]);
-                    best = fmax(best, dev::pair_cosine(gii, gjj, gij));
-                    const double rhs = eps * sqrt(fmax(gii, 0.0) * fmax(gjj, 0.0));
-                    const bool visited = !CROSS || (i < KB && j >= KB); // pairs this kernel rotates
-                    need = need || (visited && fabs(gij) > rhs && fabs(gij) > 1.0e-300);
+                    const int i = e / KW;
+                    const int j = e % KW;
+                    if (i < j)
+                    {
+                        scan(i, j);
+                    }
                 }
             }
             best = dev::wave_max(best);
@@ -858,7 +914,7 @@ namespace jsvd
                     {
                         m = fmax(m, red[w]);
                     }
-                    dev::atomic_max_nonneg(offmax, m);
+                    dev::atomic_max_nonneg(offmax, sqrt(m)); // the largest cosine
                 }
             }
         }
