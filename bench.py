@@ -50,15 +50,20 @@ def make_workload(name: str):
 
 FP32_PEAK_TFLOPS = 157.3
 
-# Flops of one Gram-space inner sweep of a 64-column block pair: 63 rounds, each updating the 528 2x2 blocks
-# of the upper triangle two-sidedly (24 flops each) and the 64 x 32 (row, pair) entries of Q (6 flops each).
-EIGEN_FLOPS_PER_PAIR = 63 * (528 * 24 + 64 * 32 * 6)
+# Flops of one inner round of the Gram-space step of a 64-column block pair: the 528 2x2 blocks of the upper
+# triangle of the 32 x 32 pair grid updated two-sidedly (24 flops each) and the 64 x 32 (row, pair) entries of Q
+# rotated (6 flops each).  The block rounds' cross ordering runs 32 such rounds per launch (pairs across the two
+# blocks only), the full cyclic sweep 63.
+EIGEN_FLOPS_PER_ROUND = 528 * 24 + 64 * 32 * 6
+EIGEN_ROUNDS_CROSS = 32
+EIGEN_FLOPS_PER_PAIR = EIGEN_ROUNDS_CROSS * EIGEN_FLOPS_PER_ROUND
 
 
-def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
-    """The mixed path's dominant kernel, block_eigen_kernel<float>, on the tall workload's round shape (16 block
-    pairs of 64 columns per round): Grams of column-graded N(0,1) 4096 x 64 blocks, every pair rotating.
-    Timed with two HIP events around `launches` back-to-back launches on the stream they run on."""
+def eigen_kernel_roofline(torch, jsvd, pairs=16, m=1024, launches=31):
+    """The mixed path's dominant kernel, block_eigen_kernel<float, 32, cross> (the cross ordering of its block
+    rounds), on the tall workload's round shape (16 block pairs of 64 columns per round of the 1024 x 1024 R^T):
+    Grams of column-graded N(0,1) 1024 x 64 blocks, every cross pair rotating.  `launches` back-to-back launches
+    enqueued from C++ (jsvd_block_eigen_ordered, repeats), timed with two HIP events on the stream they run on."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(7)
     x = torch.randn((pairs, m, 64), dtype=torch.float64, device="cuda", generator=gen)
@@ -67,13 +72,12 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
     q = torch.empty_like(gram)
     rot = torch.zeros(pairs, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
-    jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream)   # warm the code path
+    jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream, inner_order=jsvd.INNER_CROSS)   # warm the code path
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0.record(stream)
-    for _ in range(launches):
-        jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream)
+    jsvd.block_eigen(gram, 1e-7, q, rot, stream=stream, inner_order=jsvd.INNER_CROSS, repeats=launches)
     t1.record(stream)
     torch.cuda.synchronize()
     avg_us = 1000.0 * t0.elapsed_time(t1) / launches
@@ -81,11 +85,12 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
     flops = pairs * EIGEN_FLOPS_PER_PAIR
     achieved = flops / (avg_us * 1e-6) / 1e12
     return {
-        "kernel": "block_eigen_kernel<float> (Gram-space inner sweep of a block round: 63 rounds of 2x2 updates "
-                  "of G and Q, rotation parameters in FP64; one workgroup per block pair)",
+        "kernel": "block_eigen_kernel<float, 32, cross> (Gram-space inner step of a block round in the cross "
+                  "ordering: 32 rounds of 2x2 updates of G and Q, rotation parameters in FP64; one workgroup per "
+                  "block pair)",
         "bound": "latency",
         "bound_detail": "dependent chain of LDS reads, VALU updates and a barrier per inner round; issues no MFMA "
-                        "and moves 0.6 MB per launch, so neither the matrix nor the HBM roof binds it",
+                        "and moves about 0.1 MB per launch, so neither the matrix nor the HBM roof binds it",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
@@ -94,13 +99,14 @@ def eigen_kernel_roofline(torch, jsvd, pairs=16, m=4096, launches=31):
         "avg_launch_us": avg_us,
         "algorithmic_flops_per_launch": flops,
         "pairs_per_launch": pairs,
+        "inner_rounds": EIGEN_ROUNDS_CROSS,
         "rotated_pairs_per_block_pair": int(rot.float().mean().item()),
         "occupied_cus": pairs,
         "frac_of_occupied_cus": achieved / (FP32_PEAK_TFLOPS * pairs / cus),
         "note": "one 1024-thread workgroup per block pair, so one launch occupies `pairs` of the chip's CUs; a "
-                "dependent chain of 63 inner rounds bound by LDS instruction throughput and latency, far from "
-                "either roof; counted against the FP32 matrix peak of the whole chip (frac) and of the CUs it "
-                "occupies (frac_of_occupied_cus)",
+                "dependent chain of 32 inner rounds bound by LDS latency and the barrier per round, far from "
+                "either roof; counted against the FP32 peak of the whole chip (frac) and of the CUs it occupies "
+                "(frac_of_occupied_cus)",
     }
 
 
@@ -492,12 +498,20 @@ def main() -> None:
         log(f"[bench] autotune: {tune}")
 
     if args.roofline_only:
+        # The kernels whose HBM traffic the committed PMC summary reports (profiles/traffic.md): the roofline
+        # launches, one mixed and one plain solve of the workload (Gram, inner step, apply, Cholesky, transposes,
+        # round kernel), a 2,000-matrix batch under every launch shape, and a 256 MiB copy by the library's copy
+        # kernel as the counters' calibration (known bytes read and written).
         eig = eigen_kernel_roofline(torch, jsvd)
         avg_us, bytes_per_launch, _, _ = round_kernel_roofline(torch, jsvd, a_host, eps)
+        time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, 1, 0)
+        time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, a_host, None,
+                  jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=2), 1, 0)   # two sweeps: every kernel of the path
+        batch_leg(torch, jsvd, 2000, True, repeats=1, checked=0)   # every launch shape, then the tuned one
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")   # 256 MiB
         dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+        for variant in (0, 1):
+            jsvd.hbm_copy(dst, src, variant)
         torch.cuda.synchronize()
         print(json.dumps({"eigen_avg_us": eig["avg_launch_us"], "round_avg_us": avg_us, "autotune": tune,
                           "round_algorithmic_bytes": bytes_per_launch, "copy_bytes": 2 * src.numel() * 8}),

@@ -480,11 +480,13 @@ int jsvd_block_eigen(int32_t precision, int32_t block_width, const void *gram, i
 /*
  * jsvd_block_eigen in a chosen inner ordering: JSVD_INNER_FULL = the full cyclic sweep above; JSVD_INNER_CROSS
  * (block_width 32 only) = the ordering of the block rounds' default: only the 32 x 32 pairs (i, 32 + j) across
- * the two halves, in 32 inner rounds (round r pairs column u with column 32 + (u + r) mod 32).  No reference
- * counterpart.
+ * the two halves, in 32 inner rounds (round r pairs column u with column 32 + (u + r) mod 32).  The same launch
+ * is enqueued `repeats` times back to back (1..4096; each writes the same q and rotations), so it can be timed
+ * without host gaps.  No reference counterpart.
  */
 int jsvd_block_eigen_ordered(int32_t precision, int32_t block_width, int32_t inner_order, const void *gram,
-                             int64_t pairs, double epsilon, void *q, int32_t *rotations, void *stream);
+                             int64_t pairs, double epsilon, void *q, int32_t *rotations, int32_t repeats,
+                             void *stream);
 
 /* ---------------------------------------------------------------------------------------------------
  * Kernel-level entry points (device pointers, enqueued on `stream`, asynchronous).  They expose the

@@ -707,14 +707,16 @@ int jsvd_solve_f64(const double *a, int64_t m, int64_t n, const jsvd_config *cfg
 int jsvd_block_eigen(int32_t precision, int32_t block_width, const void *gram, int64_t pairs, double epsilon,
                      void *q, int32_t *rotations, void *stream)
 {
-    return jsvd_block_eigen_ordered(precision, block_width, JSVD_INNER_FULL, gram, pairs, epsilon, q, rotations,
+    return jsvd_block_eigen_ordered(precision, block_width, JSVD_INNER_FULL, gram, pairs, epsilon, q, rotations, 1,
                                     stream);
 }
 
 int jsvd_block_eigen_ordered(int32_t precision, int32_t block_width, int32_t inner_order, const void *gram,
-                             int64_t pairs, double epsilon, void *q, int32_t *rotations, void *stream)
+                             int64_t pairs, double epsilon, void *q, int32_t *rotations, int32_t repeats,
+                             void *stream)
 {
     return jsvd::guarded([&] {
+        require(repeats >= 1 && repeats <= 4096, "repeats must be in 1..4096.");
         require(inner_order == JSVD_INNER_FULL || (inner_order == JSVD_INNER_CROSS && block_width == 32),
                 "inner_order must be JSVD_INNER_FULL, or JSVD_INNER_CROSS at block_width 32.");
         require(precision == JSVD_PRECISION_F32 || precision == JSVD_PRECISION_F64, "unknown precision.");
@@ -727,15 +729,20 @@ int jsvd_block_eigen_ordered(int32_t precision, int32_t block_width, int32_t inn
         }
         require(gram && q && rotations, "buffers must not be null.");
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        if (precision == JSVD_PRECISION_F32)
+        for (int32_t k = 0; k < repeats; ++k)
         {
-            jsvd::launch_block_eigen(static_cast<const float *>(gram), block_width, static_cast<int>(pairs), epsilon,
-                                     static_cast<float *>(q), rotations, inner_order == JSVD_INNER_CROSS, s);
-        }
-        else
-        {
-            jsvd::launch_block_eigen(static_cast<const double *>(gram), block_width, static_cast<int>(pairs),
-                                     epsilon, static_cast<double *>(q), rotations, inner_order == JSVD_INNER_CROSS, s);
+            if (precision == JSVD_PRECISION_F32)
+            {
+                jsvd::launch_block_eigen(static_cast<const float *>(gram), block_width, static_cast<int>(pairs),
+                                         epsilon, static_cast<float *>(q), rotations,
+                                         inner_order == JSVD_INNER_CROSS, s);
+            }
+            else
+            {
+                jsvd::launch_block_eigen(static_cast<const double *>(gram), block_width, static_cast<int>(pairs),
+                                         epsilon, static_cast<double *>(q), rotations,
+                                         inner_order == JSVD_INNER_CROSS, s);
+            }
         }
     });
 }

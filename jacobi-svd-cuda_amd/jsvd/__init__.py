@@ -200,7 +200,7 @@ def lib() -> ctypes.CDLL:
         L.jsvd_hbm_copy.argtypes = [_vp, _vp, _i64, ctypes.c_int32, _vp]
         L.jsvd_block_eigen.argtypes = [ctypes.c_int32, ctypes.c_int32, _vp, _i64, ctypes.c_double, _vp, _vp, _vp]
         L.jsvd_block_eigen_ordered.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _i64,
-                                               ctypes.c_double, _vp, _vp, _vp]
+                                               ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp]
         L.jsvd_cholesky_f64.argtypes = [_vp, _i64, _i64, _vp, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp, _vp]
@@ -451,10 +451,11 @@ def cholesky(a, n: int, info, stream=None) -> None:
     check(lib().jsvd_cholesky_f64(_ptr(a), lda, n, _ptr(info), _stream_handle(stream)))
 
 
-def block_eigen(gram, eps: float, q, rotations, stream=None, inner_order: int = INNER_FULL) -> None:
+def block_eigen(gram, eps: float, q, rotations, stream=None, inner_order: int = INNER_FULL, repeats: int = 1) -> None:
     """One Gram-space inner sweep on each kw x kw matrix of gram (pairs x kw x kw, kw = 32, 64 or 128, float32 or
     float64 CUDA tensor): q receives the accumulated rotations, rotations (int32, pairs) the rotated-pair counts.
-    inner_order INNER_FULL: the full cyclic sweep; INNER_CROSS (kw = 64): the block rounds' cross ordering."""
+    inner_order INNER_FULL: the full cyclic sweep; INNER_CROSS (kw = 64): the block rounds' cross ordering.
+    repeats > 1 enqueues the same launch that many times back to back (for timing)."""
     import torch
     if gram.dim() != 3 or gram.shape[1] != gram.shape[2] or gram.shape[1] not in (32, 64, 128) \
             or not gram.is_contiguous():
@@ -467,7 +468,7 @@ def block_eigen(gram, eps: float, q, rotations, stream=None, inner_order: int = 
     if gram.dtype not in (torch.float32, torch.float64):
         raise ValueError("gram must be float32 or float64")
     check(lib().jsvd_block_eigen_ordered(precision, gram.shape[1] // 2, int(inner_order), _ptr(gram), gram.shape[0],
-                                         eps, _ptr(q), _ptr(rotations), _stream_handle(stream)))
+                                         eps, _ptr(q), _ptr(rotations), int(repeats), _stream_handle(stream)))
 
 
 # ------------------------------------------------------------------------------------------------

@@ -23,16 +23,17 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_traffic_figures_come_from_the_committed_summary():
-    assert bench.profiled_traffic("round_kernel_tall") == 84043737.0
-    assert bench.profiled_traffic("block_eigen_f32") == 600659.0
+    assert bench.profiled_traffic("round_kernel_tall") == 84088839.0
+    assert bench.profiled_traffic("block_eigen_f32") == 2343224.0
     assert bench.profiled_traffic("no_such_kernel") is None
 
 
 def test_eigen_flop_count():
-    # 63 inner rounds; 528 2x2 blocks of the 32 x 32 pair grid's upper triangle (24 flops each, two-sided) and
-    # 64 x 32 (row, pair) entries of Q (6 flops each)
+    # 32 inner rounds of the cross ordering; per round 528 2x2 blocks of the 32 x 32 pair grid's upper triangle
+    # (24 flops each, two-sided) and 64 x 32 (row, pair) entries of Q (6 flops each)
     assert 32 * 33 // 2 == 528
-    assert bench.EIGEN_FLOPS_PER_PAIR == 63 * (528 * 24 + 64 * 32 * 6)
+    assert bench.EIGEN_FLOPS_PER_ROUND == 528 * 24 + 64 * 32 * 6
+    assert bench.EIGEN_FLOPS_PER_PAIR == 32 * bench.EIGEN_FLOPS_PER_ROUND
 
 
 def test_tall_geometric_workload():
