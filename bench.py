@@ -209,7 +209,7 @@ def device_copy_bandwidth(torch, jsvd, mib=512, reps=5):
     return out
 
 
-def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
+def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200, compare=False):
     """Batched persistent launch over `count` uneven matrices (m ~ U{8..256}, n ~ U{4..min(m,128)}, entries
     N(0,1) x 10^U(-3,3), alternate items FP32 tol 1e-6 / FP64 tol 1e-14), inputs resident in HBM, queue in cost
     order.  The launch shape is measured by jsvd_batch_autotune first unless autotuning is off.  Accuracy of a
@@ -255,7 +255,14 @@ def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
         else:
             worst32 = max(worst32, float(np.max(np.abs(got - ref)) / ref[0]))
     best = min(times)
-    return {
+    # Algorithmic flops of the batch: per item and sweep, every column pair's three dot products (6 m) and the
+    # rotation of its A (6 m) and V (6 n) columns, over the sweeps the item ran.
+    ms_ = pk.items["rows"].astype(np.float64)
+    ns_ = pk.items["columns"].astype(np.float64)
+    sw_in = np.zeros(count)
+    sw_in[order] = sweeps
+    flops = float(np.sum(np.maximum(sw_in, 0) * ns_ * (ns_ - 1) / 2 * (12 * ms_ + 6 * ns_)))
+    out = {
         "workload": f"{count} matrices, m ~ U{{8..256}}, n ~ U{{4..min(m,128)}}, N(0,1) x 10^U(-3,3), "
                     "alternate FP32 tol 1e-6 / FP64 tol 1e-14, one persistent launch, cost-ordered queue",
         "solves_per_s": count / best,
@@ -266,6 +273,60 @@ def batch_leg(torch, jsvd, count, autotune, repeats=3, checked=200):
         "rejected": int(np.sum(sweeps < 0)),
         "accuracy_sample": {"items": min(checked, count), "against": "LAPACK gesdd (numpy.linalg.svd)",
                             "f64_sigma_relerr_max": worst64, "f32_sigma_err_rel_sigma_max": worst32},
+        "algorithmic_flops": flops,
+        "achieved_tflops": flops / best / 1e12,
+        "frac_of_fp32_vector_peak": flops / best / 1e12 / FP32_PEAK_TFLOPS,
+    }
+    if compare:
+        out.update(batch_comparison(torch, jsvd, mats, tol, order, out["solves_per_s"], shape))
+    return out
+
+
+def batch_comparison(torch, jsvd, mats, tol, order, mixed_rate, shape, plain_sample=400):
+    """The same matrices all in FP64 (tol 1e-14) through the batched kernel at the same launch shape, and a sample
+    solved one matrix at a time through a plain FP64 plan (the reference's way: one solve per matrix, re-targeted
+    with jsvd_plan_reshape, no allocation per matrix)."""
+    count = len(mats)
+    pk = jsvd.BatchPacked(mats, [jsvd.PRECISION_F64] * count, [1e-14] * count, 60)
+    d_items = torch.from_numpy(pk.items[order].view(np.uint8).copy()).to("cuda")
+    d = [torch.from_numpy(pk.a).to("cuda")]
+    d += [torch.zeros(max(x, 1), dtype=torch.float64, device="cuda") for x in pk.lens[1:]]
+    d.append(torch.zeros(count, dtype=torch.int32, device="cuda"))
+    b = jsvd.Batch(256, 128)
+    try:
+        b.set_shape(shape[0], shape[1])
+        b.solve_device(d_items, count, *d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.solve_device(d_items, count, *d)
+        all64 = time.perf_counter() - t0
+    finally:
+        b.close()
+    plan = jsvd.Plan(256, 128, jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60))
+    try:
+        sample = [mats[i] for i in range(min(plain_sample, count))]
+        dev = []
+        for a in sample:
+            m, n = a.shape
+            dev.append((torch.from_numpy(a).to("cuda"), torch.empty((m, n), dtype=torch.float64, device="cuda"),
+                        torch.empty(n, dtype=torch.float64, device="cuda"),
+                        torch.empty((n, n), dtype=torch.float64, device="cuda")))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for a, (da, du, ds, dv) in zip(sample, dev):
+            plan.reshape(a.shape[0], a.shape[1], 1e-14)
+            plan.solve_device(da, du, ds, dv)
+        plain_t = time.perf_counter() - t0
+    finally:
+        plan.close()
+    plain_rate = len(sample) / plain_t
+    return {
+        "all_fp64_batched_solves_per_s": count / all64,
+        "plain_fp64_plan_per_matrix_solves_per_s": plain_rate,
+        "plain_fp64_plan_sample": f"first {len(sample)} matrices of the batch, FP64 tol 1e-14, one plan re-targeted "
+                                  "per matrix",
+        "mixed_batched_vs_all_fp64_batched": mixed_rate / (count / all64),
+        "mixed_batched_vs_plain_fp64_per_matrix": mixed_rate / plain_rate,
     }
 
 
@@ -563,7 +624,7 @@ def main() -> None:
     batch = None
     if not args.no_batch:
         log(f"[bench] batched leg: {args.batch_count} uneven matrices in one persistent launch")
-        batch = batch_leg(torch, jsvd, args.batch_count, not args.no_autotune)
+        batch = batch_leg(torch, jsvd, args.batch_count, not args.no_autotune, compare=True)
         log(f"[bench] batched: {batch}")
 
     serving = None
