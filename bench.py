@@ -532,8 +532,8 @@ def main() -> None:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-matrices leg")
     ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving leg")
-    ap.add_argument("--serving-rate", type=float, default=1000.0)
-    ap.add_argument("--serving-seconds", type=float, default=3.0)
+    ap.add_argument("--serving-rate", type=float, default=500.0)
+    ap.add_argument("--serving-seconds", type=float, default=30.0)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernels plus a 256 MiB device copy (the counter calibration of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
@@ -631,14 +631,14 @@ def main() -> None:
     if not args.no_serving:
         from jsvd import serving as serving_mod
         log(f"[bench] serving: Poisson arrivals at {args.serving_rate}/s for {args.serving_seconds}s")
-        pool = serving_mod.request_pool(48)
+        pool = serving_mod.request_pool(96)
         srv = jsvd.Server(lanes=4, large_path=jsvd.PATH_MIXED)
         try:
             serving = serving_mod.run_poisson(srv, pool, args.serving_rate, args.serving_seconds, check_every=10)
         finally:
             srv.close()
         serving["workload"] = ("serving mix: m = exp U(ln16, ln2048), n ~ U{4..min(m,512)}, kappa 10^U(1,10), "
-                               "relerr 1e-6 / 1e-12 alternating, 48 distinct matrices cycled, 4 lanes, HBM "
+                               "relerr 1e-6 / 1e-12 alternating, 96 distinct matrices cycled, 4 lanes, HBM "
                                "budget 64 GiB, large requests on the mixed path")
         log(f"[bench] serving: {serving}")
 

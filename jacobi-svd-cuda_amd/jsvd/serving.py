@@ -29,11 +29,13 @@ def request_pool(count: int, seed: int = 4, min_m: int = 16, max_m: int = 2048, 
     return out
 
 
-def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check_every: int = 0):
+def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check_every: int = 0, keep: int = 0):
     """Submits requests from `pool` (cycled) at Poisson arrival times of `rate` per second for `duration`
     seconds, then collects every result.  Returns throughput, latency percentiles and server statistics.
     With check_every > 0, every check_every-th result's sorted sigma is compared with the pool's exact
-    singular values (normwise error)."""
+    singular values (normwise error).  With keep > 0, the results (U, sigma, V) of the first `keep` requests
+    are returned under "kept" as (pool index, u, sigma, v) for comparison with solves of the same matrices
+    elsewhere."""
     rng = np.random.default_rng(seed)
     ids = []
     t0 = time.perf_counter()
@@ -49,6 +51,7 @@ def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check
         next_t += rng.exponential(1.0 / rate)
     submit_end = time.perf_counter() - t0
     lat, queued, errs = [], [], {1e-6: 0.0, 1e-12: 0.0}
+    kept = []
     bad = 0
     for j, (rid, p) in enumerate(ids):
         u, s, v, info = server.wait(rid)
@@ -57,6 +60,8 @@ def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check
             continue
         lat.append(info.latency_s)
         queued.append(info.queued_s)
+        if j < keep:
+            kept.append((p, u, s, v))
         if check_every and j % check_every < 2:  # consecutive pairs: both requested accuracies
             _, rel, exact = pool[p]
             err = float(np.max(np.abs(np.sort(s)[::-1] - exact)) / exact[0])
@@ -64,7 +69,7 @@ def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check
     makespan = time.perf_counter() - t0
     lat_ms = np.array(lat) * 1e3
     stats = server.stats()
-    return {
+    rep = {
         "rate_target": rate,
         "submitted": len(ids),
         "offered_rate": len(ids) / submit_end if submit_end > 0 else 0.0,
@@ -78,6 +83,9 @@ def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check
         "max_sigma_err_fp64_requests": errs[1e-12] if check_every else None,
         "server": stats,
     }
+    if keep:
+        rep["kept"] = kept
+    return rep
 
 
 def main(argv=None) -> None:

@@ -93,3 +93,34 @@ def test_poisson_serving_short_run(torch_cuda):
     assert rep["max_sigma_err_fp32_requests"] < 1e-5
     assert rep["max_sigma_err_fp64_requests"] < 1e-12
     assert rep["server"]["max_in_flight"] > 1
+
+
+def test_poisson_serving_workload_at_500_per_s(torch_cuda):
+    """The serving workload at its configured shapes (m = exp U(ln 16, ln 2048), n ~ U{4..min(m, 512)},
+    kappa = 10^U(1, 10), relerr 1e-6 / 1e-12 alternating) offered at 500 requests/s: every request completes,
+    the server keeps up (completed rate within 10 % of the offered rate), each precision class meets its
+    accuracy (normwise sigma error vs the construction's exact values: 1e-5 for relerr 1e-6, 1e-12 for relerr
+    1e-12, checked on every request), and the first results equal, bit for bit, the same matrices served alone."""
+    from jsvd import serving
+    pool = serving.request_pool(32, seed=11, max_m=2048, max_n=512)
+    assert max(a.shape[0] for a, _, _ in pool) > 1024 and max(a.shape[1] for a, _, _ in pool) > 256
+    srv = jsvd.Server(lanes=4, large_path=jsvd.PATH_MIXED)
+    try:
+        print("[c4] 500/s for 3 s over 32 matrices up to 2048 x 512", flush=True)
+        rep = serving.run_poisson(srv, pool, rate=500.0, duration=3.0, check_every=1, keep=16)
+    finally:
+        srv.close()
+    assert rep["failed"] == 0 and rep["completed"] == rep["submitted"] > 1200
+    assert rep["solves_per_s"] > 0.9 * rep["offered_rate"], rep
+    assert rep["max_sigma_err_fp32_requests"] < 1e-5
+    assert rep["max_sigma_err_fp64_requests"] < 1e-12
+    assert rep["server"]["batches"] > 0 and rep["server"]["plans"] > 0 and rep["server"]["max_in_flight"] > 1
+    alone = jsvd.Server(lanes=1, large_path=jsvd.PATH_MIXED)
+    try:
+        for p, u, s, v in rep["kept"]:
+            a, rel, _ = pool[p]
+            u1, s1, v1, info1 = alone.wait(alone.submit(a, rel))
+            assert info1.status == 0
+            assert np.array_equal(s, s1) and np.array_equal(u, u1) and np.array_equal(v, v1), (a.shape, rel)
+    finally:
+        alone.close()
