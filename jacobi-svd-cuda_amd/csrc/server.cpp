@@ -345,6 +345,7 @@ struct jsvd_server
     {
         (void)hipSetDevice(device);
         LaneState ls;
+        std::string init_error; // this lane's own: read by no other thread
         try
         {
             JSVD_HIP_CHECK(hipStreamCreateWithFlags(&ls.stream, hipStreamNonBlocking));
@@ -352,8 +353,7 @@ struct jsvd_server
         }
         catch (const std::exception &e)
         {
-            std::lock_guard<std::mutex> lock(mu);
-            lane_error = e.what();
+            init_error = e.what();
         }
         for (;;)
         {
@@ -392,7 +392,7 @@ struct jsvd_server
             {
                 if (!ls.batch)
                 {
-                    throw jsvd::Error(JSVD_ERR_DEVICE, "lane initialisation failed: " + lane_error);
+                    throw jsvd::Error(JSVD_ERR_DEVICE, "lane initialisation failed: " + init_error);
                 }
                 if (job.size() == 1 && !job[0]->small)
                 {
@@ -422,8 +422,6 @@ struct jsvd_server
             finish(job);
         }
     }
-
-    std::string lane_error;
 };
 
 extern "C" {

@@ -29,52 +29,94 @@ def request_pool(count: int, seed: int = 4, min_m: int = 16, max_m: int = 2048, 
     return out
 
 
-def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check_every: int = 0, keep: int = 0):
+def run_poisson(server, pool, rate: float, duration: float, seed: int = 0, check_every: int = 0, keep: int = 0,
+                max_outstanding: int = 20000):
     """Submits requests from `pool` (cycled) at Poisson arrival times of `rate` per second for `duration`
-    seconds, then collects every result.  Returns throughput, latency percentiles and server statistics.
-    With check_every > 0, every check_every-th result's sorted sigma is compared with the pool's exact
-    singular values (normwise error).  With keep > 0, the results (U, sigma, V) of the first `keep` requests
-    are returned under "kept" as (pool index, u, sigma, v) for comparison with solves of the same matrices
-    elsewhere."""
+    seconds while a collector thread takes the results in submission order.  Returns throughput, latency
+    percentiles and server statistics.
+
+    An arrival that finds `max_outstanding` requests submitted but not yet collected is shed (counted, not
+    submitted): past the server's capacity the backlog, and the host memory the queued matrices hold, stay
+    bounded.  With check_every > 0, every check_every-th result's sorted sigma is compared with the pool's
+    exact singular values (normwise error).  With keep > 0, the results (U, sigma, V) of the first `keep`
+    requests are returned under "kept" as (pool index, u, sigma, v) for comparison with solves of the same
+    matrices elsewhere."""
+    import queue
+    import threading
+
     rng = np.random.default_rng(seed)
-    ids = []
+    lat, queued, errs = [], [], {1e-6: 0.0, 1e-12: 0.0}
+    kept = []
+    bad = [0]
+    collected = [0]
+    todo = queue.Queue()
+    failure = []
+
+    def collect():
+        j = 0
+        while True:
+            item = todo.get()
+            if item is None:
+                return
+            rid, p = item
+            try:
+                u, s, v, info = server.wait(rid)
+            except Exception as e:  # reported after the run; the loop keeps draining
+                failure.append(e)
+                bad[0] += 1
+                collected[0] += 1
+                j += 1
+                continue
+            if info.status != 0:
+                bad[0] += 1
+            else:
+                lat.append(info.latency_s)
+                queued.append(info.queued_s)
+                if j < keep:
+                    kept.append((p, u, s, v))
+                if check_every and j % check_every < 2:  # consecutive pairs: both requested accuracies
+                    _, rel, exact = pool[p]
+                    err = float(np.max(np.abs(np.sort(s)[::-1] - exact)) / exact[0])
+                    errs[rel] = max(errs[rel], err)
+            collected[0] += 1
+            j += 1
+
+    th = threading.Thread(target=collect, daemon=True)
+    th.start()
+    submitted = shed = 0
     t0 = time.perf_counter()
     next_t = rng.exponential(1.0 / rate)
     k = 0
-    while next_t < duration:
-        now = time.perf_counter() - t0
-        if now < next_t:
-            time.sleep(next_t - now)
-        a, rel, _ = pool[k % len(pool)]
-        ids.append((server.submit(a, rel), k % len(pool)))
-        k += 1
-        next_t += rng.exponential(1.0 / rate)
-    submit_end = time.perf_counter() - t0
-    lat, queued, errs = [], [], {1e-6: 0.0, 1e-12: 0.0}
-    kept = []
-    bad = 0
-    for j, (rid, p) in enumerate(ids):
-        u, s, v, info = server.wait(rid)
-        if info.status != 0:
-            bad += 1
-            continue
-        lat.append(info.latency_s)
-        queued.append(info.queued_s)
-        if j < keep:
-            kept.append((p, u, s, v))
-        if check_every and j % check_every < 2:  # consecutive pairs: both requested accuracies
-            _, rel, exact = pool[p]
-            err = float(np.max(np.abs(np.sort(s)[::-1] - exact)) / exact[0])
-            errs[rel] = max(errs[rel], err)
+    try:
+        while next_t < duration:
+            now = time.perf_counter() - t0
+            if now < next_t:
+                time.sleep(next_t - now)
+            if submitted - collected[0] >= max_outstanding:
+                shed += 1
+            else:
+                a, rel, _ = pool[k % len(pool)]
+                todo.put((server.submit(a, rel), k % len(pool)))
+                submitted += 1
+            k += 1
+            next_t += rng.exponential(1.0 / rate)
+        submit_end = time.perf_counter() - t0
+    finally:
+        todo.put(None)
+        th.join()
+    if failure:
+        raise failure[0]
     makespan = time.perf_counter() - t0
     lat_ms = np.array(lat) * 1e3
     stats = server.stats()
     rep = {
         "rate_target": rate,
-        "submitted": len(ids),
-        "offered_rate": len(ids) / submit_end if submit_end > 0 else 0.0,
+        "arrivals": k,
+        "submitted": submitted,
+        "shed": shed,
+        "offered_rate": k / submit_end if submit_end > 0 else 0.0,
         "completed": len(lat),
-        "failed": bad,
+        "failed": bad[0],
         "solves_per_s": len(lat) / makespan if makespan > 0 else 0.0,
         "p50_ms": float(np.percentile(lat_ms, 50)) if lat else None,
         "p99_ms": float(np.percentile(lat_ms, 99)) if lat else None,
@@ -103,6 +145,8 @@ def main(argv=None) -> None:
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--large-path", default="mixed", choices=["plain", "mixed"])
     ap.add_argument("--check-every", type=int, default=25)
+    ap.add_argument("--max-outstanding", type=int, default=20000,
+                    help="arrivals beyond this many uncollected requests are shed")
     ap.add_argument("--small-rows", type=int, default=256, help="largest m served by the batched kernel")
     ap.add_argument("--small-cols", type=int, default=128, help="largest n served by the batched kernel")
     args = ap.parse_args(argv)
@@ -113,7 +157,8 @@ def main(argv=None) -> None:
         srv = jsvd.Server(lanes=args.lanes, large_path=large, small_max_rows=args.small_rows,
                           small_max_columns=args.small_cols)
         try:
-            rep = run_poisson(srv, pool, rate, args.duration, check_every=args.check_every)
+            rep = run_poisson(srv, pool, rate, args.duration, check_every=args.check_every,
+                              max_outstanding=args.max_outstanding)
         finally:
             srv.close()
         rep["lanes"] = args.lanes

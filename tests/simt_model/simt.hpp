@@ -24,11 +24,12 @@
 #include <vector>
 
 // simt_switch(save, to): saves the callee-saved registers on the current stack, stores the stack pointer in
-// *save, switches to stack `to` and restores the registers saved there.
+// *save, switches to stack `to` and restores the registers saved there.  A weak symbol: every translation unit
+// that includes this header emits the same definition and the linker keeps one.
 extern "C" void simt_switch(void **save, void *to);
 asm(R"(
     .text
-    .globl simt_switch
+    .weak simt_switch
     .type simt_switch, @function
 simt_switch:
     pushq %rbp
