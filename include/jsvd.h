@@ -571,6 +571,38 @@ int jsvd_u_sigma_f64(const double *a, int64_t lda, int64_t m, int64_t n, double 
  */
 int jsvd_hbm_copy(void *dst, const void *src, int64_t bytes, int32_t variant, void *stream);
 
+/*
+ * The FP64 dense kernels of the mixed path's QR-first preconditioning and final products, as stand-alone
+ * device-pointer entry points (enqueued on `stream`, a hipStream_t; they return once the work is done).  No
+ * reference counterpart: the reference has no dense linear algebra (it sweeps A itself).
+ *
+ * jsvd_gemm_f64: d = alpha op(a) op(b) + beta c, op(x) = x or x^T (trans_a / trans_b nonzero), op(a) m x k, op(b)
+ * k x n, column-major a (lda), b (ldb); c and d column-major m x n (ldc, ldd) unless JSVD_GEMM_ROW_MAJOR_CD.  c may
+ * equal d; it is not read when beta == 0.  flags restrict the work to the structure:
+ *   JSVD_GEMM_UPPER_OUT   only entries (i, j) in 64 x 64 tiles on or above the diagonal are written (m == n)
+ *   JSVD_GEMM_MIRROR      with UPPER_OUT: the tiles above the diagonal are also written transposed below it
+ *                         (a symmetric product in full)
+ *   JSVD_GEMM_TRI_A       op(a) is upper triangular: its entries below the diagonal are not read
+ *   JSVD_GEMM_TRI_B       op(b) is upper triangular: likewise
+ *   JSVD_GEMM_ZERO_LOWER  entries of d below the diagonal are written as zero
+ *   JSVD_GEMM_ROW_MAJOR_CD c and d row-major (entry (i, j) at i * ld + j)
+ * Results do not depend on the launch's timing (split-k partials are summed in a fixed order).
+ *
+ * jsvd_trsm_right_upper_f64: b (rows x n, column-major, ldb) <- b r^-1 for the upper triangle of r (n x n,
+ * column-major, ldr; its strictly lower triangle is not read).
+ */
+#define JSVD_GEMM_UPPER_OUT 1
+#define JSVD_GEMM_MIRROR 2
+#define JSVD_GEMM_TRI_A 4
+#define JSVD_GEMM_TRI_B 8
+#define JSVD_GEMM_ZERO_LOWER 16
+#define JSVD_GEMM_ROW_MAJOR_CD 32
+int jsvd_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, double alpha, const double *a,
+                  int64_t lda, const double *b, int64_t ldb, double beta, const double *c, int64_t ldc, double *d,
+                  int64_t ldd, int32_t flags, void *stream);
+int jsvd_trsm_right_upper_f64(double *b, int64_t ldb, int64_t rows, int64_t n, const double *r, int64_t ldr,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif

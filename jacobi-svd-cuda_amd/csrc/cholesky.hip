@@ -41,6 +41,7 @@ namespace jsvd
                                                                                int k, int *__restrict__ info)
         {
             __shared__ __align__(16) double row[2][kT];
+            __shared__ int tile_first_bad; // first step of this tile with a non-positive pivot (kT: none)
             const int l = threadIdx.x % kT;
             const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kT)); // wave-uniform
             const int64_t r0 = static_cast<int64_t>(k) * kT;
@@ -51,7 +52,11 @@ namespace jsvd
             {
                 x[i] = col[i];
             }
-            bool bad = false;
+            if (threadIdx.x == 0)
+            {
+                tile_first_bad = kT; // ordered before every update by the first step's barrier
+            }
+            int first_bad = kT;
 #pragma unroll
             for (int c = 0; c < kT; ++c)
             {
@@ -63,7 +68,7 @@ namespace jsvd
                     double piv = lane_value(x[oi], c);
                     if (!(piv > 0.0))
                     {
-                        bad = true;
+                        first_bad = (first_bad == kT) ? c : first_bad;
                         piv = 1.0; // keep the arithmetic finite; the factorisation is reported as failed
                     }
                     const double r = sqrt(piv);
@@ -99,9 +104,15 @@ namespace jsvd
                     col[i] = x[i];
                 }
             }
-            if (l == 0 && bad)
+            if (l == 0 && first_bad < kT)
             {
-                atomicCAS(info, 0, k * kT + 1);
+                atomicMin(&tile_first_bad, first_bad);
+            }
+            __syncthreads();
+            // tiles are factored in order, so the first tile to fail sets info: 1 + its first bad row
+            if (threadIdx.x == 0 && tile_first_bad < kT)
+            {
+                atomicCAS(info, 0, k * kT + tile_first_bad + 1);
             }
         }
 

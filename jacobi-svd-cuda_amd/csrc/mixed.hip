@@ -9,15 +9,14 @@
 //   X = M V64 (FP64 GEMM); FP64 block sweeps on (X, V64) to epsilon
 //   sigma_j = |x_j|, V_A = X / sigma (row-major), U_A = Q V64 written row-major by one GEMM, columns with
 //   sigma_j <= epsilon zeroed as the reference zeroes them
-// The GEMMs, triangular solves and products are plain library calls (rocBLAS dgemm / dtrsm / dtrmm); the
-// Cholesky factorisations are this library's tiled kernels (cholesky.hip).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
+// The products (B^T B and V^T V on their upper triangle only, R_k R_acc on triangular operands) and the triangular
+// solves are this library's FP64 MFMA kernels (dense.hip), as are the tiled Cholesky factorisations
+// (cholesky.hip).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
 // Yamamoto and Yanagisawa, "Shifted Cholesky QR for computing the QR factorization of ill-conditioned
 // matrices" (SIAM J. Sci. Comput. 42, 2020): shift s = 11 (m n + n (n + 1)) u |B|^2 with |B|_F^2 <= n for
 // unit columns; the column scaling keeps the factorisation's backward error columnwise, so column-graded
 // matrices keep their relative accuracy.
 #include "mixed.hpp"
-
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -27,21 +26,13 @@
 
 #include "autotune.hpp"
 #include "cholesky.hpp"
+#include "dense.hpp"
 #include "plain_f64.hpp"
 
 namespace jsvd
 {
     namespace
     {
-        void blas_check(rocblas_status st, const char *what)
-        {
-            if (st != rocblas_status_success)
-            {
-                throw Error(JSVD_ERR_DEVICE, std::string("rocBLAS call failed: ") + what + ", status=" +
-                                                 rocblas_status_to_string(st));
-            }
-        }
-
         double default_fp32_tolerance(int64_t rows)
         {
             return std::max(1e-6, 4.0 * std::sqrt(static_cast<double>(rows)) * std::ldexp(1.0, -24));
@@ -213,20 +204,16 @@ namespace jsvd
             JSVD_HIP_CHECK(hipMalloc(&v32_, sizeof(float) * v_elems));
             JSVD_HIP_CHECK(hipMemsetAsync(a64b_, 0, sizeof(double) * a_elems, stream_));
             JSVD_HIP_CHECK(hipMemsetAsync(w64_, 0, sizeof(double) * v_elems, stream_));
-            rocblas_handle h = nullptr;
-            blas_check(rocblas_create_handle(&h), "rocblas_create_handle");
-            blas_ = h;
-            blas_check(rocblas_set_stream(h, stream_), "rocblas_set_stream");
+            // scratch of the dense kernels: inverted diagonal blocks of the triangular solve and split-k partial tiles
+            JSVD_HIP_CHECK(hipMalloc(&dinv_, sizeof(double) * trsm_dinv_elems(ga_.n_pad)));
+            dws_.work_elems = dense_work_elems();
+            JSVD_HIP_CHECK(hipMalloc(&dws_.work, sizeof(double) * dws_.work_elems));
         }
     }
 
     BlockSolver::~BlockSolver()
     {
-        if (blas_)
-        {
-            (void)rocblas_destroy_handle(static_cast<rocblas_handle>(blas_));
-        }
-        for (void *p : {static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
+        for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
                         static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(off_),
@@ -316,21 +303,18 @@ namespace jsvd
     // orthogonality error of ~1e-6 to FP64 rounding in two steps.
     void BlockSolver::orthogonalize_v(double *v, double *w)
     {
-        rocblas_handle h = static_cast<rocblas_handle>(blas_);
-        const rocblas_int np = static_cast<rocblas_int>(g_.n_pad);
-        const rocblas_int ldv = static_cast<rocblas_int>(g_.ldv);
+        const int64_t np = g_.n_pad;
+        const int64_t ldv = g_.ldv;
         double *cur = v;
         double *nxt = w;
         for (int step = 0; step < mixed_.orthogonalize_steps; ++step)
         {
-            const double one = 1.0, zero = 0.0, minus_half = -0.5, three_halves = 1.5;
-            blas_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, np, np, np, &one, cur, ldv,
-                                     cur, ldv, &zero, gram64_, np),
-                       "dgemm V^T V");
-            JSVD_HIP_CHECK(hipMemcpyAsync(nxt, cur, sizeof(double) * g_.ldv * g_.n_pad, hipMemcpyDeviceToDevice, stream_));
-            blas_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, np, np, np, &minus_half, cur,
-                                     ldv, gram64_, np, &three_halves, nxt, ldv),
-                       "dgemm V (V^T V)");
+            // G = V^T V (upper tiles, mirrored), then nxt = 1.5 cur - 0.5 cur G over all ldv rows (padding rows of
+            // cur are zero and stay zero)
+            launch_gemm_f64(true, false, np, np, np, 1.0, cur, ldv, cur, ldv, 0.0, nullptr, 0, gram64_, np,
+                            dense::kUpperOut | dense::kMirror, dws_, stream_);
+            launch_gemm_f64(false, false, ldv, np, np, -0.5, cur, ldv, gram64_, np, 1.5, cur, ldv, nxt, ldv, 0, dws_,
+                            stream_);
             std::swap(cur, nxt);
         }
         if (cur != v)
@@ -384,34 +368,27 @@ namespace jsvd
         info.sweeps_fp32 = run_sweeps<float>(g, a32_, v32_, tol, fp32_cap, rot_, off_, info.offmax_fp32);
         launch_convert_f32_to_f64(v32_, v64_, v_elems, stream_);
         orthogonalize_v(v64_, w64_);
-        rocblas_handle h = static_cast<rocblas_handle>(blas_);
-        const double one = 1.0, zero = 0.0;
-        blas_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, static_cast<rocblas_int>(g.lda),
-                                 static_cast<rocblas_int>(g.n_pad), static_cast<rocblas_int>(g.n_pad), &one, x64,
-                                 static_cast<rocblas_int>(g.lda), v64_, static_cast<rocblas_int>(g.ldv), &zero, y64,
-                                 static_cast<rocblas_int>(g.lda)),
-                   "dgemm X V");
+        launch_gemm_f64(false, false, g.lda, g.n_pad, g.n_pad, 1.0, x64, g.lda, v64_, g.ldv, 0.0, nullptr, 0, y64,
+                        g.lda, 0, dws_, stream_);
     }
 
     bool BlockSolver::factor_qr()
     {
-        rocblas_handle h = static_cast<rocblas_handle>(blas_);
-        const rocblas_int m = static_cast<rocblas_int>(ga_.m);
-        const rocblas_int n = static_cast<rocblas_int>(ga_.n);
-        const rocblas_int lda = static_cast<rocblas_int>(ga_.lda);
-        const rocblas_int ldr = static_cast<rocblas_int>(cholesky_padded_size(ga_.n_pad));
+        const int64_t m = ga_.m;
+        const int64_t n = ga_.n;
+        const int64_t lda = ga_.lda;
+        const int64_t ldr = cholesky_padded_size(ga_.n_pad);
         hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(ga_.n_pad)), dim3(256), 0, stream_, a64_,
                            ga_.lda, ga_.m, ga_.n, dcol_);
         JSVD_HIP_CHECK(hipGetLastError());
         // |B|_2^2 <= |B|_F^2 <= n for unit columns
         const double u = std::ldexp(1.0, -53);
         const double shift = 11.0 * (static_cast<double>(m) * n + static_cast<double>(n) * (n + 1)) * u * n;
-        const double one = 1.0, zero = 0.0;
         for (int pass = 0; pass < 3; ++pass)
         {
-            blas_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, m, &one, a64_, lda,
-                                     a64_, lda, &zero, gram64_, ldr),
-                       "dgemm B^T B");
+            // G = B^T B on the upper triangle (the only part the factorisation reads)
+            launch_gemm_f64(true, false, n, n, m, 1.0, a64_, lda, a64_, lda, 0.0, nullptr, 0, gram64_, ldr,
+                            dense::kUpperOut, dws_, stream_);
             if (pass == 0)
             {
                 hipLaunchKernelGGL(add_diagonal_kernel, dim3(grid_of(n)), dim3(256), 0, stream_, gram64_,
@@ -419,9 +396,8 @@ namespace jsvd
                 JSVD_HIP_CHECK(hipGetLastError());
             }
             launch_cholesky_upper(gram64_, ldr, n, chol_info_ + pass, stream_);
-            blas_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                     rocblas_diagonal_non_unit, m, n, &one, gram64_, ldr, a64_, lda),
-                       "dtrsm B R^-1");
+            // B <- B R^-1 (the factor's identity padding up to ldr is the padding the solve needs)
+            launch_trsm_right_upper_f64(a64_, lda, m, n, gram64_, ldr, true, nullptr, dinv_, dws_, stream_);
             if (pass == 0)
             {
                 hipLaunchKernelGGL(copy_upper_kernel, dim3(grid_of(static_cast<int64_t>(n) * n)), dim3(256), 0,
@@ -432,9 +408,8 @@ namespace jsvd
             else
             {
                 // racc <- R_pass racc (upper times upper stays upper), through w64_ (free until the sweeps)
-                blas_check(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                                         rocblas_diagonal_non_unit, n, n, &one, gram64_, ldr, racc_, ldr, w64_, ldr),
-                           "dtrmm R racc");
+                launch_gemm_f64(false, false, n, n, n, 1.0, gram64_, ldr, racc_, ldr, 0.0, nullptr, 0, w64_, ldr,
+                                dense::kTriA | dense::kTriB | dense::kZeroLower, dws_, stream_);
                 JSVD_HIP_CHECK(hipMemcpyAsync(racc_, w64_, sizeof(double) * ldr * n, hipMemcpyDeviceToDevice,
                                               stream_));
             }
@@ -470,13 +445,8 @@ namespace jsvd
         // sigma_j = |x_j| and V_A = X / sigma (row-major), no cutoff: a column is normalised whenever sigma_j > 0
         launch_u_sigma(w64_, g_.lda, g_.n, g_.n, 0.0, d_v, d_sigma, stream_);
         // U_A = Q V_M, produced row-major directly: U^T = V_M^T Q^T is column-major n x m with leading dimension n
-        rocblas_handle h = static_cast<rocblas_handle>(blas_);
-        const double one = 1.0, zero = 0.0;
-        const rocblas_int m = static_cast<rocblas_int>(ga_.m), n = static_cast<rocblas_int>(ga_.n);
-        blas_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, n, m, n, &one, v64_,
-                                 static_cast<rocblas_int>(g_.ldv), a64_, static_cast<rocblas_int>(ga_.lda), &zero, d_u,
-                                 n),
-                   "dgemm (Q V)^T");
+        launch_gemm_f64(false, false, ga_.m, ga_.n, ga_.n, 1.0, a64_, ga_.lda, v64_, g_.ldv, 0.0, nullptr, 0, d_u,
+                        ga_.n, dense::kRowMajorCD, dws_, stream_);
         hipLaunchKernelGGL(mask_columns_kernel, dim3(grid_of(ga_.m * ga_.n)), dim3(256), 0, stream_, d_u, ga_.m,
                            ga_.n, static_cast<const double *>(d_sigma), cfg_.epsilon);
         JSVD_HIP_CHECK(hipGetLastError());

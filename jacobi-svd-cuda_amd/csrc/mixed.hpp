@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "block.hpp"
+#include "dense.hpp"
 
 namespace jsvd
 {
@@ -59,7 +60,8 @@ namespace jsvd
         jsvd_config cfg_;
         jsvd_mixed_config mixed_;
         hipStream_t stream_;
-        void *blas_ = nullptr; // rocblas_handle
+        DenseWorkspace dws_;         // split-k scratch of the dense products
+        double *dinv_ = nullptr;     // inverted diagonal blocks of the triangular solve
         double *a64_ = nullptr;
         double *a64b_ = nullptr;
         double *v64_ = nullptr;

@@ -202,6 +202,9 @@ def lib() -> ctypes.CDLL:
         L.jsvd_block_eigen_ordered.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _i64,
                                                ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp]
         L.jsvd_cholesky_f64.argtypes = [_vp, _i64, _i64, _vp, _vp]
+        L.jsvd_gemm_f64.argtypes = [ctypes.c_int32, ctypes.c_int32, _i64, _i64, _i64, ctypes.c_double, _vp, _i64,
+                                    _vp, _i64, ctypes.c_double, _vp, _i64, _vp, _i64, ctypes.c_int32, _vp]
+        L.jsvd_trsm_right_upper_f64.argtypes = [_vp, _i64, _i64, _i64, _vp, _i64, _vp]
         L.jsvd_rounds_f64.argtypes = [_vp, _i64, _i64, _vp, _i64, _i64, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp, _vp]
         for name in declared_symbols():
@@ -435,6 +438,23 @@ def hbm_copy(dst, src, variant: int = 0, stream=None) -> None:
     if dst.numel() * dst.element_size() != nbytes:
         raise ValueError("dst and src must have the same byte size")
     check(lib().jsvd_hbm_copy(_ptr(dst), _ptr(src), nbytes, int(variant), _stream_handle(stream)))
+
+
+GEMM_UPPER_OUT, GEMM_MIRROR, GEMM_TRI_A, GEMM_TRI_B, GEMM_ZERO_LOWER, GEMM_ROW_MAJOR_CD = 1, 2, 4, 8, 16, 32
+
+
+def gemm(trans_a: bool, trans_b: bool, m: int, n: int, k: int, alpha: float, a, lda: int, b, ldb: int, beta: float,
+         c, ldc: int, d, ldd: int, flags: int = 0, stream=None) -> None:
+    """jsvd_gemm_f64 on float64 CUDA tensors (flat storage, column-major with the given leading dimensions; c may
+    be None when beta == 0, or the same tensor as d)."""
+    check(lib().jsvd_gemm_f64(int(bool(trans_a)), int(bool(trans_b)), m, n, k, float(alpha), _ptr(a), lda, _ptr(b),
+                              ldb, float(beta), _ptr(c) if c is not None else None, ldc, _ptr(d), ldd, int(flags),
+                              _stream_handle(stream)))
+
+
+def trsm_right_upper(b, ldb: int, rows: int, n: int, r, ldr: int, stream=None) -> None:
+    """jsvd_trsm_right_upper_f64: b <- b r^-1 on float64 CUDA tensors (column-major, flat storage)."""
+    check(lib().jsvd_trsm_right_upper_f64(_ptr(b), ldb, rows, n, _ptr(r), ldr, _stream_handle(stream)))
 
 
 def cholesky(a, n: int, info, stream=None) -> None:
