@@ -172,6 +172,24 @@ int jsvd_plan_last_info(const jsvd_plan *plan, jsvd_solve_info *info);
 int jsvd_plan_reshape(jsvd_plan *plan, int64_t m, int64_t n, double epsilon);
 
 /*
+ * Round spans of a plain FP64 plan: the sweep's rounds run as an even number of launches of up to max_rounds
+ * rounds each instead of one launch per round (src/kernels.cu:889-981 launches the two kernels of every round
+ * from the host).  Each workgroup owns a range of the round-robin circle's slots and also computes the slots that
+ * feed them (temporal blocking with redundant halos), one wavefront per slot, the columns in registers, so
+ * workgroups never wait for each other; the launches alternate between two buffers.  Results are bit-identical to
+ * one launch per round.  waves = 8 or 16 wavefronts per workgroup, or 0 for one launch per round.  Plans choose
+ * spans by themselves (the autotuner's measured choice, else spans for m n <= 2^19 where the build has the
+ * kernel); this call overrides that until the next reshape.  Fails (JSVD_ERR_INVALID_ARGUMENT) when the shape
+ * has no span kernel (m <= 1024, n <= 512, n >= 3 with the plan's round shape), when the plan's capacity is above
+ * 1024 x 512 (no second buffers), or for block / mixed plans.
+ */
+int jsvd_plan_set_round_span(jsvd_plan *plan, int32_t waves, int32_t max_rounds);
+
+/* The plan's current round-span setting: waves (0 = one launch per round), rounds per launch at most, and the
+ * launches one sweep takes. */
+int jsvd_plan_round_span(const jsvd_plan *plan, int32_t *waves, int32_t *max_rounds, int32_t *launches_per_sweep);
+
+/*
  * The launch parameters a plan currently uses: for the plain path the fused round kernel's threads per column
  * pair and register chunk counts (0 = streamed), block fields 0; for the block paths the apply kernel's total
  * workgroups and the block width, round fields 0.  They come from the mixed config's block_width, from
@@ -438,6 +456,11 @@ typedef struct jsvd_tune_report
                                       workgroups; 0 where the width leaves fewer than two blocks) */
     double block_round_us_default; /* us per FP32 block round at the chosen width, 512 apply workgroups */
     double block_round_us_best;    /* us per FP32 block round at the chosen width and workgroup count */
+    int32_t span_waves;            /* plain path: wavefronts per round-span workgroup chosen (0 = one launch
+                                      per round) */
+    int32_t span_rounds;           /* plain path: rounds per span launch at most (0 = one launch per round) */
+    double sweep_us_rounds;        /* plain path: us per sweep with one launch per round (chosen shape) */
+    double sweep_us_span;          /* plain path: us per sweep with the chosen spans (0 when none was timed) */
 } jsvd_tune_report;
 
 /*

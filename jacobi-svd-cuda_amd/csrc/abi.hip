@@ -34,6 +34,48 @@ namespace jsvd
         Tuning t;
         return (lookup_tuning(m, n, t) && t.has_round) ? t.round : choose_round_shape(m, n, false);
     }
+
+    // Largest m x n whose plans keep the second pair of buffers the round spans alternate with (register-resident
+    // columns make spans possible only up to m <= 1024, n <= 512 anyway).
+    constexpr int64_t kSpanMaxElems = int64_t{1024} * 512;
+
+    // Round-span setting for a plain plan of m x n: the autotuner's measured choice (round shape and spans) when
+    // it has one, else, for m n <= 2^19 where the build has the kernel, spans of up to 4 rounds on 8-wavefront
+    // workgroups with the 64-thread round shape (unless a layout mode is forced): one wavefront per pair then
+    // reduces in the order of one 64-lane tree.  Measured on one MI355X, spans of 4 rounds on 8 wavefronts beat
+    // one launch per round from 64 x 32 (1.7x) to 1000 x 500 (1.14x); 16-wavefront workgroups, whose cones hold
+    // 16 slots, were slower on every shape.
+    void plan_span(int64_t m, int64_t n, const jsvd_config &cfg, RoundShape &shape, int &waves, int &rounds)
+    {
+        waves = 0;
+        rounds = 0;
+        Tuning t;
+        if (lookup_tuning(m, n, t) && t.has_round && t.round.threads == shape.threads && t.round.ea == shape.ea &&
+            t.round.ev == shape.ev)
+        {
+            if (t.span_waves > 0 && span_supported(shape, t.span_waves, m, n))
+            {
+                waves = t.span_waves;
+                rounds = t.span_rounds;
+            }
+            return;
+        }
+        if (m * n > (int64_t{1} << 19))
+        {
+            return;
+        }
+        RoundShape cand = shape;
+        if (cfg.layout_transpose_mode == JSVD_LAYOUT_AUTO)
+        {
+            cand = choose_round_shape(m, n, false);
+        }
+        if (span_supported(cand, 8, m, n))
+        {
+            shape = cand;
+            waves = 8;
+            rounds = std::min(span_max_rounds(8, n), 4);
+        }
+    }
 } // namespace jsvd
 
 // The opaque plan: one shape, its device workspace, a stream, and the pinned mailbox through which the
@@ -52,6 +94,17 @@ struct jsvd_plan
     bool own_stream = false;
     double *a_col = nullptr;            // lda x n column-major working copy of A
     double *v_col = nullptr;            // ldv x n column-major V
+    // Round spans (several rounds per launch): wavefronts per workgroup (0 = one launch per round), rounds per
+    // launch at most, the slot partition and the rounds of each launch of a sweep; the spans alternate between
+    // (a_col, v_col) and the second buffers (capacity-sized, allocated when the capacity allows spans).
+    int span_waves = 0;
+    int span_rounds = 0;
+    int span_groups = 0;
+    int span_own = 0;
+    int span_own_end = 0;
+    std::vector<int> span_chunk;
+    double *a_alt = nullptr;
+    double *v_alt = nullptr;
     int *rot_count = nullptr;           // [max_sweeps] rotated pairs per sweep
     unsigned long long *offmax = nullptr; // [max_sweeps] max pair cosine per sweep (double bits)
     // per-sweep counter slots the round kernels update (folded into rot_count / offmax after each sweep)
@@ -79,6 +132,8 @@ struct jsvd_plan
         block.reset();
         (void)hipFree(a_col);
         (void)hipFree(v_col);
+        (void)hipFree(a_alt);
+        (void)hipFree(v_alt);
         (void)hipFree(rot_count);
         (void)hipFree(offmax);
         (void)hipFree(rot_slots);
@@ -143,6 +198,11 @@ struct jsvd_plan
         }
         JSVD_HIP_CHECK(hipMalloc(&a_col, sizeof(double) * lda * n));
         JSVD_HIP_CHECK(hipMalloc(&v_col, sizeof(double) * ldv * n));
+        if (lda * n <= jsvd::kSpanMaxElems)
+        {
+            JSVD_HIP_CHECK(hipMalloc(&a_alt, sizeof(double) * lda * n));
+            JSVD_HIP_CHECK(hipMalloc(&v_alt, sizeof(double) * ldv * n));
+        }
         JSVD_HIP_CHECK(hipMalloc(&rot_count, sizeof(int) * cfg.max_sweeps));
         JSVD_HIP_CHECK(hipMalloc(&offmax, sizeof(unsigned long long) * cfg.max_sweeps));
         JSVD_HIP_CHECK(hipMalloc(&rot_slots, sizeof(int) * kCountSlots * cfg.max_sweeps));
@@ -155,6 +215,43 @@ struct jsvd_plan
         // Padding rows m..lda-1 stay zero for the plan's lifetime: transposes only write rows < m and the
         // round kernel reads/writes the padding row of an odd m as zero.
         JSVD_HIP_CHECK(hipMemsetAsync(a_col, 0, sizeof(double) * lda * n, stream));
+        configure_span();
+    }
+
+    // Applies the span rule (plan_span) to the current shape; spans need the second buffers.
+    void configure_span()
+    {
+        int waves = 0, rounds = 0;
+        if (a_alt != nullptr)
+        {
+            jsvd::plan_span(m, n, cfg, shape, waves, rounds);
+        }
+        set_span(waves, rounds);
+    }
+
+    // Spans of at most `rounds` rounds on `waves`-wavefront workgroups (waves == 0: one launch per round).
+    void set_span(int waves, int rounds)
+    {
+        span_waves = 0;
+        span_rounds = 0;
+        span_chunk.clear();
+        if (waves == 0)
+        {
+            return;
+        }
+        if (a_alt == nullptr || !jsvd::span_supported(shape, waves, m, n))
+        {
+            jsvd::throw_invalid("round spans are not available for this plan's shape.");
+        }
+        rounds = std::min(rounds, jsvd::span_max_rounds(waves, n));
+        if (rounds < 2)
+        {
+            jsvd::throw_invalid("round spans need at least 2 rounds per launch.");
+        }
+        jsvd::span_partition(n, waves, rounds, span_groups, span_own, span_own_end);
+        span_chunk = jsvd::span_chunks(n, rounds);
+        span_waves = waves;
+        span_rounds = rounds;
     }
 
     // Re-targets the plan at rows x cols with epsilon `eps` without reallocating; false (plan unchanged) when the
@@ -182,9 +279,14 @@ struct jsvd_plan
             ldv = jsvd::round_up(cols, 2);
             shape = jsvd::plan_round_shape(rows, cols, cfg);
         }
+        const bool changed = rows != m || cols != n;
         m = rows;
         n = cols;
         cfg.epsilon = eps;
+        if (!block && changed)
+        {
+            configure_span();
+        }
         return true;
     }
 
@@ -232,10 +334,28 @@ struct jsvd_plan
         for (int s = 0; s < max_sweeps; ++s)
         {
             const int *skip = (s > 0) ? rot_count + (s - 1) : nullptr;
-            for (int r = 0; r < rounds; ++r)
+            if (span_waves > 0)
             {
-                jsvd::launch_round(shape, a_col, lda, m, v_col, ldv, n, r, eps, rot_slots + s * kCountSlots,
-                                   off_slots + s * kCountSlots, skip, stream, kCountSlots);
+                // an even number of span launches, alternating buffers: every sweep ends in (a_col, v_col), and
+                // the launches of a skipped sweep write nothing
+                int r0 = 0;
+                for (size_t i = 0; i < span_chunk.size(); ++i)
+                {
+                    const bool even = (i % 2) == 0;
+                    jsvd::launch_round_span(shape, span_waves, even ? a_col : a_alt, even ? a_alt : a_col, lda, m,
+                                            even ? v_col : v_alt, even ? v_alt : v_col, ldv, n, r0, span_chunk[i],
+                                            span_groups, span_own, span_own_end, eps, rot_slots + s * kCountSlots,
+                                            off_slots + s * kCountSlots, skip, stream, kCountSlots);
+                    r0 += span_chunk[i];
+                }
+            }
+            else
+            {
+                for (int r = 0; r < rounds; ++r)
+                {
+                    jsvd::launch_round(shape, a_col, lda, m, v_col, ldv, n, r, eps, rot_slots + s * kCountSlots,
+                                       off_slots + s * kCountSlots, skip, stream, kCountSlots);
+                }
             }
             jsvd::launch_sweep_totals(rot_slots + s * kCountSlots, off_slots + s * kCountSlots, kCountSlots,
                                       rot_count + s, offmax + s, stream);
@@ -508,6 +628,28 @@ int jsvd_plan_create_ex(int64_t m, int64_t n, const jsvd_config *cfg, const jsvd
         require(plan != nullptr, "plan output pointer must not be null.");
         *plan = nullptr;
         *plan = make_plan(m, n, cfg, mixed, static_cast<hipStream_t>(stream)).release();
+    });
+}
+
+int jsvd_plan_set_round_span(jsvd_plan *plan, int32_t waves, int32_t max_rounds)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr, "plan must not be null.");
+        require(!plan->block, "round spans belong to the plain path.");
+        require(waves == 0 || waves == 8 || waves == 16, "waves must be 0, 8 or 16.");
+        plan->set_span(waves, max_rounds);
+    });
+}
+
+int jsvd_plan_round_span(const jsvd_plan *plan, int32_t *waves, int32_t *max_rounds, int32_t *launches_per_sweep)
+{
+    return jsvd::guarded([&] {
+        require(plan != nullptr && waves && max_rounds && launches_per_sweep, "plan and outputs must not be null.");
+        *waves = plan->span_waves;
+        *max_rounds = plan->span_rounds;
+        const int64_t N = plan->n + (plan->n & 1);
+        *launches_per_sweep = plan->span_waves > 0 ? static_cast<int32_t>(plan->span_chunk.size())
+                                                   : static_cast<int32_t>(plan->n >= 2 ? N - 1 : 0);
     });
 }
 

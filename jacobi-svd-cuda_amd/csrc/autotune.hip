@@ -142,6 +142,37 @@ namespace jsvd
             return ev.elapsed_us() / rounds;
         }
 
+        // Microseconds per sweep of round spans (waves, rounds) of `shape`: one warm-up sweep, then one timed
+        // sweep, alternating between (a, v) and (a2, v2) as the plan does.
+        double time_span_sweep(const RoundShape &shape, int waves, int rounds, double *a, double *a2, int64_t lda,
+                               int64_t m, double *v, double *v2, int64_t ldv, int64_t n, int *slots,
+                               unsigned long long *off, hipStream_t stream)
+        {
+            int groups = 0, own = 0, own_end = 0;
+            span_partition(n, waves, rounds, groups, own, own_end);
+            const std::vector<int> chunks = span_chunks(n, rounds);
+            Events ev;
+            for (int rep = 0; rep < 2; ++rep)
+            {
+                if (rep == 1)
+                {
+                    JSVD_HIP_CHECK(hipEventRecord(ev.a, stream));
+                }
+                int r0 = 0;
+                for (size_t i = 0; i < chunks.size(); ++i)
+                {
+                    const bool even = (i % 2) == 0;
+                    launch_round_span(shape, waves, even ? a : a2, even ? a2 : a, lda, m, even ? v : v2, even ? v2 : v,
+                                      ldv, n, r0, chunks[i], groups, own, own_end, 1e-15, slots, off, nullptr, stream,
+                                      64);
+                    r0 += chunks[i];
+                }
+            }
+            JSVD_HIP_CHECK(hipEventRecord(ev.b, stream));
+            JSVD_HIP_CHECK(hipEventSynchronize(ev.b));
+            return ev.elapsed_us();
+        }
+
         // Microseconds per FP32 block round of g on seeded data: up to 8 rounds of a first sweep, best of
         // `repetitions`; with the cross ordering also the intra-block step of a sweep (intra_us).
         double time_block_rounds(const BlockGeometry &g, int repetitions, hipStream_t stream, double &intra_us)
@@ -280,6 +311,8 @@ namespace jsvd
             if (t.has_round)
             {
                 e.round = t.round;
+                e.span_waves = t.span_waves;
+                e.span_rounds = t.span_rounds;
                 e.has_round = true;
             }
             if (t.has_block)
@@ -321,17 +354,24 @@ namespace jsvd
             JSVD_HIP_CHECK(hipMemsetAsync(slots.p, 0, sizeof(int) * 64, stream));
             JSVD_HIP_CHECK(hipMemsetAsync(off.p, 0, sizeof(unsigned long long) * 64, stream));
             const int rounds = 16;
-            double best_us = -1.0;
+            const int per_sweep = static_cast<int>(n + (n & 1) - 1);
+            const bool span_room = lda * n <= int64_t{1024} * 512;
+            DevBuf a2(span_room ? sizeof(double) * lda * n : 8), v2(span_room ? sizeof(double) * ldv * n : 8);
+            // per candidate shape: one launch per round (16 rounds, scaled to a sweep) and, where the build has
+            // span kernels for it, whole sweeps of spans; the fastest (shape, spans) per sweep wins
+            double best_sweep = -1.0;
             for (const RoundShape &cand : round_candidates(m, n))
             {
                 // same data for every candidate: A from the seeded fill (padding rows zero), V = I
                 // (the kernel reads the padding row of an odd m as zero whatever it holds)
+                auto reset = [&] {
+                    JSVD_HIP_CHECK(hipMemcpyAsync(a.p, a0.p, sizeof(double) * lda * n, hipMemcpyDeviceToDevice, stream));
+                    launch_identity(static_cast<double *>(v.p), ldv, n, stream);
+                };
                 double us = -1.0;
                 for (int r = 0; r < repetitions; ++r)
                 {
-                    JSVD_HIP_CHECK(
-                        hipMemcpyAsync(a.p, a0.p, sizeof(double) * lda * n, hipMemcpyDeviceToDevice, stream));
-                    launch_identity(static_cast<double *>(v.p), ldv, n, stream);
+                    reset();
                     const double t = time_rounds(cand, static_cast<double *>(a.p), lda, m, static_cast<double *>(v.p),
                                                  ldv, n, static_cast<int *>(slots.p),
                                                  static_cast<unsigned long long *>(off.p), rounds, stream);
@@ -343,14 +383,67 @@ namespace jsvd
                 {
                     rep.round_us_default = us;
                 }
-                if (best_us < 0.0 || us < best_us)
+                if (best_sweep < 0.0 || us * per_sweep < best_sweep)
                 {
-                    best_us = us;
+                    best_sweep = us * per_sweep;
                     best.round = cand;
+                    best.span_waves = 0;
+                    best.span_rounds = 0;
+                    rep.round_us_best = us;
+                    rep.sweep_us_rounds = us * per_sweep;
+                }
+                if (!span_room)
+                {
+                    continue;
+                }
+                for (int waves : {8, 16})
+                {
+                    if (!span_supported(cand, waves, m, n))
+                    {
+                        continue;
+                    }
+                    const int most = span_max_rounds(waves, n);
+                    std::vector<int> cands;
+                    for (int r : {2, 3, 4, 6, 8})
+                    {
+                        if (r < most)
+                        {
+                            cands.push_back(r);
+                        }
+                    }
+                    cands.push_back(most);
+                    for (int r : cands)
+                    {
+                        double sw = -1.0;
+                        for (int rep_i = 0; rep_i < repetitions; ++rep_i)
+                        {
+                            reset();
+                            const double t = time_span_sweep(
+                                cand, waves, r, static_cast<double *>(a.p), static_cast<double *>(a2.p), lda, m,
+                                static_cast<double *>(v.p), static_cast<double *>(v2.p), ldv, n,
+                                static_cast<int *>(slots.p), static_cast<unsigned long long *>(off.p), stream);
+                            sw = (sw < 0.0) ? t : std::min(sw, t);
+                        }
+                        ++rep.candidates_tried;
+                        if (rep.sweep_us_span <= 0.0 || sw < rep.sweep_us_span)
+                        {
+                            rep.sweep_us_span = sw;
+                        }
+                        if (sw < best_sweep)
+                        {
+                            best_sweep = sw;
+                            best.round = cand;
+                            best.span_waves = waves;
+                            best.span_rounds = r;
+                            rep.round_us_best = us;
+                            rep.sweep_us_rounds = us * per_sweep;
+                        }
+                    }
                 }
             }
-            rep.round_us_best = best_us;
             best.has_round = true;
+            rep.span_waves = best.span_waves;
+            rep.span_rounds = best.span_rounds;
         }
         // block paths over m x n (the block FP64 path and the mixed path's direct form)
         const BlockChoice bc = tune_block(m, n, repetitions, stream);

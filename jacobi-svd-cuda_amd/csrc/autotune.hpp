@@ -9,6 +9,8 @@ namespace jsvd
     struct Tuning
     {
         RoundShape round{};   // plain path: round-kernel launch shape (valid when has_round)
+        int span_waves = 0;   // plain path: round spans on this many wavefronts per workgroup (0: one launch per round)
+        int span_rounds = 0;  // plain path: rounds per span launch at most
         int apply_wgs = 512;  // block sweeps over m x n: total workgroups of one apply launch (has_block)
         int block_width = 32; // block sweeps over m x n: columns per block (has_block)
         int batch_shape = 0;  // batched solver with limits m x n: index of its launch shape (has_batch)

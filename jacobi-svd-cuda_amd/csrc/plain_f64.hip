@@ -13,6 +13,7 @@
 // 16-byte boundary and rows move as double2), V is column-major (n rows, ldv even).  Rows m..lda-1 are
 // workspace padding and are read as zero.
 #include "plain_f64.hpp"
+#include "plain_kernels.hpp"
 
 #include <algorithm>
 
@@ -20,6 +21,9 @@ namespace jsvd
 {
     namespace
     {
+        using plain::block_sum3;
+        using plain::round_kernel;
+
         constexpr int kTile = 64;      // transpose tile edge (doubles)
         constexpr int kTileRows = 4;   // block is kTile x kTileRows threads
 
@@ -106,245 +110,6 @@ namespace jsvd
             const int64_t col = idx / ldv;
             const int64_t row = idx % ldv;
             v[idx] = (row == col) ? 1.0 : 0.0;
-        }
-
-        // Block-wide sum of three doubles.  Deterministic: a fixed DPP row-move tree inside each wavefront (no
-        // LDS crossbar round trips on the reduction's dependent chain), then the per-wave totals are added in
-        // wave order.  Every thread returns the totals.
-        template <int T>
-        __device__ inline void block_sum3(double &x, double &y, double &z, double (*scratch)[3])
-        {
-            x = dev::wave_sum_dpp(x);
-            y = dev::wave_sum_dpp(y);
-            z = dev::wave_sum_dpp(z);
-            if constexpr (T > 64)
-            {
-                constexpr int W = T / 64;
-                const int wave = threadIdx.x / 64;
-                if ((threadIdx.x % 64) == 0)
-                {
-                    scratch[wave][0] = x;
-                    scratch[wave][1] = y;
-                    scratch[wave][2] = z;
-                }
-                __syncthreads();
-                double sx = 0.0, sy = 0.0, sz = 0.0;
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                {
-                    sx += scratch[w][0];
-                    sy += scratch[w][1];
-                    sz += scratch[w][2];
-                }
-                x = sx;
-                y = sy;
-                z = sz;
-            }
-        }
-
-        __device__ inline void rotate2(double2 &x, double2 &y, double c, double s)
-        {
-            const double2 px = x;
-            const double2 py = y;
-            x.x = c * px.x - s * py.x;
-            x.y = c * px.y - s * py.y;
-            y.x = s * px.x + c * py.x;
-            y.y = s * px.y + c * py.y;
-        }
-
-        // Streams a column pair through registers applying (c, s); the padding half of an odd length is
-        // read and written as zero.
-        __device__ inline void rotate_stream(double2 *xp, double2 *yp, int64_t len2, bool odd_tail, double c,
-                                             double s, int tid, int T)
-        {
-            for (int64_t idx = tid; idx < len2; idx += T)
-            {
-                double2 x = xp[idx];
-                double2 y = yp[idx];
-                if (odd_tail && idx == len2 - 1)
-                {
-                    x.y = 0.0;
-                    y.y = 0.0;
-                }
-                rotate2(x, y, c, s);
-                xp[idx] = x;
-                yp[idx] = y;
-            }
-        }
-
-        // Register-resident fused round.  Each workgroup owns one slot of the round-robin circle, i.e. one
-        // column pair; EA / EV double2 chunks per thread hold the pair's A and V columns.  The grid covers
-        // the N/2 slots of the round (N = n rounded up to even).  EA == 0 selects the streaming variant for
-        // columns that do not fit in registers: a Gram pass, then an apply pass that re-reads the columns.
-        template <int T, int EA, int EV>
-        __global__ __launch_bounds__(T) void round_kernel(double *__restrict__ a, int64_t lda, int64_t m,
-                                                          double *__restrict__ v, int64_t ldv, int64_t n, int N,
-                                                          int round, double eps, int *__restrict__ rot_count,
-                                                          unsigned long long *__restrict__ offmax,
-                                                          const int *__restrict__ skip_if_zero, int count_slots)
-        {
-            __shared__ double scratch[T / 64 > 0 ? T / 64 : 1][3];
-            // the pair's slot of the rotation counter and the cosine maximum (count_slots is a power of two):
-            // spreading the one atomic per workgroup over slots keeps same-address atomics from serialising
-            const int cslot = static_cast<int>(blockIdx.x) & (count_slots - 1);
-            if (skip_if_zero != nullptr && *skip_if_zero == 0)
-            {
-                return;
-            }
-            const int slot = blockIdx.x;
-            const int p = dev::circle_at(slot, round, N);
-            const int q = dev::circle_at(N - 1 - slot, round, N);
-            if (p >= n || q >= n)
-            {
-                return; // the dummy column of an odd column count
-            }
-            const int tid = threadIdx.x;
-            const int64_t m2 = (m + 1) / 2;   // double2 chunks holding the m rows
-            const bool odd_tail = (m & 1) != 0; // the .y of chunk m2-1 is padding
-            const int64_t n2 = (n + 1) / 2;
-            const bool odd_tail_v = (n & 1) != 0;
-            double2 *colp = reinterpret_cast<double2 *>(a + static_cast<int64_t>(p) * lda);
-            double2 *colq = reinterpret_cast<double2 *>(a + static_cast<int64_t>(q) * lda);
-            double2 *vp = reinterpret_cast<double2 *>(v + static_cast<int64_t>(p) * ldv);
-            double2 *vq = reinterpret_cast<double2 *>(v + static_cast<int64_t>(q) * ldv);
-
-            double spp = 0.0, sqq = 0.0, spq = 0.0;
-            if constexpr (EA > 0)
-            {
-                double2 ap[EA], aq[EA], xv[EV > 0 ? EV : 1], yv[EV > 0 ? EV : 1];
-#pragma unroll
-                for (int k = 0; k < EA; ++k)
-                {
-                    const int64_t idx = static_cast<int64_t>(k) * T + tid;
-                    if (idx < m2)
-                    {
-                        ap[k] = colp[idx];
-                        aq[k] = colq[idx];
-                        if (odd_tail && idx == m2 - 1)
-                        {
-                            ap[k].y = 0.0;
-                            aq[k].y = 0.0;
-                        }
-                    }
-                    else
-                    {
-                        ap[k] = make_double2(0.0, 0.0);
-                        aq[k] = make_double2(0.0, 0.0);
-                    }
-                }
-                // V columns are requested before the reduction so their latency overlaps it (EV == 0: V is
-                // streamed after the rotation decision instead, for shapes whose registers are full).
-#pragma unroll
-                for (int k = 0; k < EV; ++k)
-                {
-                    const int64_t idx = static_cast<int64_t>(k) * T + tid;
-                    if (idx < n2)
-                    {
-                        xv[k] = vp[idx];
-                        yv[k] = vq[idx];
-                        if (odd_tail_v && idx == n2 - 1)
-                        {
-                            xv[k].y = 0.0;
-                            yv[k].y = 0.0;
-                        }
-                    }
-                    else
-                    {
-                        xv[k] = make_double2(0.0, 0.0);
-                        yv[k] = make_double2(0.0, 0.0);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < EA; ++k)
-                {
-                    spp += ap[k].x * ap[k].x + ap[k].y * ap[k].y;
-                    sqq += aq[k].x * aq[k].x + aq[k].y * aq[k].y;
-                    spq += ap[k].x * aq[k].x + ap[k].y * aq[k].y;
-                }
-                block_sum3<T>(spp, sqq, spq, scratch);
-                double c, s;
-                const bool rotate = dev::rotation_params(spp, sqq, spq, eps, c, s);
-                if (tid == 0)
-                {
-                    if (offmax)
-                    {
-                        dev::atomic_max_nonneg(offmax + cslot, dev::pair_cosine(spp, sqq, spq));
-                    }
-                    if (rotate && rot_count)
-                    {
-                        atomicAdd(rot_count + cslot, 1);
-                    }
-                }
-                if (!rotate)
-                {
-                    return; // c = 1, s = 0 leaves both columns bit-identical: skip the stores
-                }
-#pragma unroll
-                for (int k = 0; k < EA; ++k)
-                {
-                    const int64_t idx = static_cast<int64_t>(k) * T + tid;
-                    if (idx < m2)
-                    {
-                        rotate2(ap[k], aq[k], c, s);
-                        colp[idx] = ap[k];
-                        colq[idx] = aq[k];
-                    }
-                }
-                if constexpr (EV > 0)
-                {
-#pragma unroll
-                    for (int k = 0; k < EV; ++k)
-                    {
-                        const int64_t idx = static_cast<int64_t>(k) * T + tid;
-                        if (idx < n2)
-                        {
-                            rotate2(xv[k], yv[k], c, s);
-                            vp[idx] = xv[k];
-                            vq[idx] = yv[k];
-                        }
-                    }
-                }
-                else
-                {
-                    rotate_stream(vp, vq, n2, odd_tail_v, c, s, tid, T);
-                }
-            }
-            else
-            {
-                for (int64_t idx = tid; idx < m2; idx += T)
-                {
-                    double2 x = colp[idx];
-                    double2 y = colq[idx];
-                    if (odd_tail && idx == m2 - 1)
-                    {
-                        x.y = 0.0;
-                        y.y = 0.0;
-                    }
-                    spp += x.x * x.x + x.y * x.y;
-                    sqq += y.x * y.x + y.y * y.y;
-                    spq += x.x * y.x + x.y * y.y;
-                }
-                block_sum3<T>(spp, sqq, spq, scratch);
-                double c, s;
-                const bool rotate = dev::rotation_params(spp, sqq, spq, eps, c, s);
-                if (tid == 0)
-                {
-                    if (offmax)
-                    {
-                        dev::atomic_max_nonneg(offmax + cslot, dev::pair_cosine(spp, sqq, spq));
-                    }
-                    if (rotate && rot_count)
-                    {
-                        atomicAdd(rot_count + cslot, 1);
-                    }
-                }
-                if (!rotate)
-                {
-                    return;
-                }
-                rotate_stream(colp, colq, m2, odd_tail, c, s, tid, T);
-                rotate_stream(vp, vq, n2, odd_tail_v, c, s, tid, T);
-            }
         }
 
         // Folds the counter slots of one sweep: one 64-lane wave, slots <= 64 x 64.
@@ -613,6 +378,144 @@ namespace jsvd
         }
         hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(N / 2)), dim3(static_cast<unsigned>(shape.threads)), 0,
                            stream, a, lda, m, v, ldv, n, N, round, eps, rot_count, offmax, skip_if_zero, count_slots);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
+    namespace
+    {
+        using SpanFn = void (*)(const double *, double *, int64_t, int64_t, const double *, double *, int64_t,
+                                int64_t, int, int, int, int, int, double, int *, unsigned long long *, const int *,
+                                int);
+
+        // The span kernel for (TT, EA, NV, W), or null where the build has none: V chunks beyond the A chunks
+        // (n <= m), or a stage of W columns that would not fit in LDS.
+        template <int TT, int EA, int NV, int W>
+        SpanFn span_fn()
+        {
+            constexpr int CA = (TT / 64) * EA;
+            // (16 wavefronts get 128 registers per lane: at most 8 chunks of each of the two columns there)
+            if constexpr (NV <= CA && W * (CA + NV) * 1024 <= 160 * 1024 && (W < 16 || CA + NV <= 8))
+            {
+                return plain::round_span_kernel<TT, EA, NV, W>;
+            }
+            return nullptr;
+        }
+
+        template <int TT, int EA, int W>
+        SpanFn pick_span_nv(int nv)
+        {
+            switch (nv)
+            {
+            case 1: return span_fn<TT, EA, 1, W>();
+            case 2: return span_fn<TT, EA, 2, W>();
+            case 4: return span_fn<TT, EA, 4, W>();
+            default: return nullptr;
+            }
+        }
+
+        template <int W>
+        SpanFn pick_span_w(int threads, int ea, int nv)
+        {
+            if (threads == 64)
+            {
+                switch (ea)
+                {
+                case 1: return pick_span_nv<64, 1, W>(nv);
+                case 2: return pick_span_nv<64, 2, W>(nv);
+                case 4: return pick_span_nv<64, 4, W>(nv);
+                case 8: return pick_span_nv<64, 8, W>(nv);
+                default: return nullptr;
+                }
+            }
+            if (threads == 256)
+            {
+                switch (ea)
+                {
+                case 1: return pick_span_nv<256, 1, W>(nv);
+                case 2: return pick_span_nv<256, 2, W>(nv);
+                default: return nullptr;
+                }
+            }
+            return nullptr;
+        }
+
+        // V chunks per lane of a column: a power of two covering ceil(n / 2) double2 chunks in 64 lanes.
+        int span_nv(int64_t n)
+        {
+            const int64_t per = ((n + 1) / 2 + 63) / 64;
+            return per <= 1 ? 1 : (per <= 2 ? 2 : (per <= 4 ? 4 : 0));
+        }
+
+        SpanFn pick_span(const RoundShape &shape, int waves, int64_t n)
+        {
+            const int nv = span_nv(n);
+            if (nv == 0 || shape.ea < 1)
+            {
+                return nullptr;
+            }
+            const int ca = (shape.threads / 64) * shape.ea;
+            if (ca > 8 || nv > ca)
+            {
+                return nullptr;
+            }
+            if (waves == 16 && ca + nv <= 8)
+            {
+                return pick_span_w<16>(shape.threads, shape.ea, nv);
+            }
+            if (waves == 8)
+            {
+                return pick_span_w<8>(shape.threads, shape.ea, nv);
+            }
+            return nullptr;
+        }
+    } // namespace
+
+    bool span_supported(const RoundShape &shape, int waves, int64_t m, int64_t n)
+    {
+        if (n < 3 || m < n)
+        {
+            return false;
+        }
+        const int64_t m2 = (m + 1) / 2;
+        if (shape.ea < 1 || static_cast<int64_t>(shape.ea) * shape.threads < m2)
+        {
+            return false;
+        }
+        return pick_span(shape, waves, n) != nullptr;
+    }
+
+    int span_max_rounds(int waves, int64_t n)
+    {
+        const int N = static_cast<int>(n + (n & 1));
+        const int H = N / 2;
+        return H <= waves ? N - 1 : (waves + 1) / 2;
+    }
+
+    void span_partition(int64_t n, int waves, int max_rounds, int &groups, int &own, int &own_end)
+    {
+        plain::span_partition(n, waves, max_rounds, groups, own, own_end);
+    }
+
+    std::vector<int> span_chunks(int64_t n, int max_rounds) { return plain::span_chunks(n, max_rounds); }
+
+    void launch_round_span(const RoundShape &shape, int waves, const double *a_in, double *a_out, int64_t lda,
+                           int64_t m, const double *v_in, double *v_out, int64_t ldv, int64_t n, int r0, int rounds,
+                           int groups, int own, int own_end, double eps, int *rot_count, unsigned long long *offmax,
+                           const int *skip_if_zero, hipStream_t stream, int count_slots)
+    {
+        if (count_slots < 1 || (count_slots & (count_slots - 1)) != 0)
+        {
+            throw_invalid("count_slots must be a power of two.");
+        }
+        SpanFn fn = span_supported(shape, waves, m, n) ? pick_span(shape, waves, n) : nullptr;
+        if (fn == nullptr || rounds < 1)
+        {
+            throw_invalid("round span: unsupported launch shape.");
+        }
+        const int N = static_cast<int>(n + (n & 1));
+        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(groups)), dim3(static_cast<unsigned>(64 * waves)), 0,
+                           stream, a_in, a_out, lda, m, v_in, v_out, ldv, n, N, r0, rounds, own, own_end, eps,
+                           rot_count, offmax, skip_if_zero, count_slots);
         JSVD_HIP_CHECK(hipGetLastError());
     }
 

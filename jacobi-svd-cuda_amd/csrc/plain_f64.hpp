@@ -3,6 +3,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace jsvd
 {
     // Launch shape of the fused round kernel: `threads` per workgroup (one workgroup per column pair) and the
@@ -26,6 +28,25 @@ namespace jsvd
     void launch_round(const RoundShape &shape, double *a, int64_t lda, int64_t m, double *v, int64_t ldv, int64_t n,
                       int round, double eps, int *rot_count, unsigned long long *offmax, const int *skip_if_zero,
                       hipStream_t stream, int count_slots = 1);
+
+    // Round span (several rounds of a sweep per launch, bit-identical to one launch per round with `shape`):
+    // whether the build has a span kernel for `shape` at `waves` wavefronts per workgroup (8 or 16) and this
+    // m x n (register-resident columns: m <= 1024, n <= 512, n >= 3).
+    bool span_supported(const RoundShape &shape, int waves, int64_t m, int64_t n);
+    // Most rounds one span launch can run: the whole sweep when one workgroup holds every slot, else the rounds
+    // whose halo still leaves each workgroup a slot of its own.
+    int span_max_rounds(int waves, int64_t n);
+    // Workgroups and owned slots per workgroup for spans of at most max_rounds rounds (throws when the cone of a
+    // workgroup would not fit).
+    void span_partition(int64_t n, int waves, int max_rounds, int &groups, int &own, int &own_end);
+    // Rounds per launch over one sweep: an even number of launches of at most max_rounds rounds.
+    std::vector<int> span_chunks(int64_t n, int max_rounds);
+    // Rounds [r0, r0 + rounds) from (a_in, v_in) into (a_out, v_out); the columns must be in both buffers'
+    // layout (lda, ldv).  Counter slots as launch_round.
+    void launch_round_span(const RoundShape &shape, int waves, const double *a_in, double *a_out, int64_t lda,
+                           int64_t m, const double *v_in, double *v_out, int64_t ldv, int64_t n, int r0, int rounds,
+                           int groups, int own, int own_end, double eps, int *rot_count, unsigned long long *offmax,
+                           const int *skip_if_zero, hipStream_t stream, int count_slots);
 
     // rot_total = sum of rot_slots[0..slots), off_total = max of off_slots[0..slots) (one workgroup).
     void launch_sweep_totals(const int *rot_slots, const unsigned long long *off_slots, int slots, int *rot_total,

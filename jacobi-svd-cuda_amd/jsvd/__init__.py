@@ -328,6 +328,21 @@ class Plan:
         return dict(zip(("round_threads", "round_ea", "round_ev", "apply_workgroups", "block_width"),
                         (v.value for v in vals)))
 
+    def set_round_span(self, waves: int, max_rounds: int = 8) -> None:
+        """Round spans of a plain plan: `waves` = 8 or 16 wavefronts per workgroup and up to max_rounds rounds per
+        launch, or waves = 0 for one launch per round (jsvd_plan_set_round_span)."""
+        L = lib()
+        L.jsvd_plan_set_round_span.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
+        check(L.jsvd_plan_set_round_span(self._h, int(waves), int(max_rounds)))
+
+    def round_span(self) -> dict:
+        """The plan's round-span setting (jsvd_plan_round_span)."""
+        L = lib()
+        L.jsvd_plan_round_span.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int32)] * 3
+        vals = [ctypes.c_int32() for _ in range(3)]
+        check(L.jsvd_plan_round_span(self._h, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(("waves", "max_rounds", "launches_per_sweep"), (v.value for v in vals)))
+
     def reshape(self, m: int, n: int, epsilon: float) -> None:
         """Re-targets the plan at m x n and epsilon inside its allocated capacity (jsvd_plan_reshape)."""
         L = lib()
@@ -877,6 +892,10 @@ class TuneReport(ctypes.Structure):
         ("block_sweep_us", ctypes.c_double * 3),
         ("block_round_us_default", ctypes.c_double),
         ("block_round_us_best", ctypes.c_double),
+        ("span_waves", ctypes.c_int32),
+        ("span_rounds", ctypes.c_int32),
+        ("sweep_us_rounds", ctypes.c_double),
+        ("sweep_us_span", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
