@@ -513,13 +513,20 @@ namespace jsvd
             __shared__ double rb[64][65]; // rb[k][j] = R_bb(k, j)
             const int j = threadIdx.x;
             const int64_t d0 = static_cast<int64_t>(blockIdx.x) * 64;
+            double x[64];
+            // column j of the block: thread j reads row k of it (R_bb(k, j)); all 64 loads in flight before the
+            // first wait (the padded R holds the whole 64 x 64 block, so the loads below the diagonal stay inside it)
+#pragma unroll
             for (int k = 0; k < 64; ++k)
             {
-                // column j of the block: thread j reads row k of it (R_bb(k, j)); the rows are 64 consecutive columns
-                rb[k][j] = (k <= j) ? rp[(d0 + k) + (d0 + j) * ld] : 0.0;
+                x[k] = rp[(d0 + k) + (d0 + j) * ld];
+            }
+#pragma unroll
+            for (int k = 0; k < 64; ++k)
+            {
+                rb[k][j] = (k <= j) ? x[k] : 0.0;
             }
             __syncthreads();
-            double x[64];
 #pragma unroll
             for (int i = 63; i >= 0; --i)
             {

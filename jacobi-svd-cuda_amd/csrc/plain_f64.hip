@@ -38,19 +38,34 @@ namespace jsvd
             const int64_t bx = static_cast<int64_t>(blockIdx.x) % tiles_x;
             const int64_t by = static_cast<int64_t>(blockIdx.x) / tiles_x;
             const int64_t c = bx * kTile + tid_x;
+            // every load of the tile in flight before the first wait: unconditional loads (element 0 outside the
+            // matrix), the bounds applied to the LDS stores
+            constexpr int kPer = kTile / kTileRows;
+            double x[kPer];
+            double sr[Scale ? kPer : 1];
 #pragma unroll
-            for (int j = 0; j < kTile; j += kTileRows)
+            for (int j = 0; j < kPer; ++j)
             {
-                const int64_t r = by * kTile + tid_y + j;
+                const int64_t r = by * kTile + tid_y + j * kTileRows;
+                const bool in = r < rows && c < cols;
+                x[j] = src[in ? r * lds + c : 0];
+                if constexpr (Scale)
+                {
+                    sr[j] = row_scale[in ? r : 0];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kPer; ++j)
+            {
+                const int64_t r = by * kTile + tid_y + j * kTileRows;
                 if (r < rows && c < cols)
                 {
-                    double x = src[r * lds + c];
+                    double v = x[j];
                     if constexpr (Scale)
                     {
-                        const double sr = row_scale[r];
-                        x = (sr > eps) ? (x / sr) : 0.0;
+                        v = (sr[j] > eps) ? (v / sr[j]) : 0.0;
                     }
-                    tile[tid_y + j][tid_x] = x;
+                    tile[tid_y + j * kTileRows][tid_x] = v;
                 }
             }
             __syncthreads();
