@@ -528,7 +528,7 @@ def main() -> None:
     ap.add_argument("--workload", default="tall", choices=["tall", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison leg")
-    ap.add_argument("--cpu-reps", type=int, default=3, help="full CPU-baseline solves of the workload (median)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="full CPU-baseline solves of the workload (median)")
     ap.add_argument("--no-file", action="store_true", help="skip the streamed *.mat file leg")
     ap.add_argument("--block-width", type=int, default=0, choices=[0, 16, 32, 64],
                     help="block width of the mixed path (0: the autotuner's choice)")
