@@ -133,6 +133,13 @@ typedef struct jsvd_mixed_config
                                     the two blocks (32 inner rounds), plus one step per sweep that sweeps every
                                     block by itself; JSVD_INNER_FULL = one cyclic sweep over all 64 columns of the
                                     block pair (63 inner rounds).  Other widths always use the full sweep. */
+    double fp32_stop_fraction;   /* mixed path: the FP32 phase (a preconditioner of the FP64 phase) ends after the
+                                    first sweep that rotates at most this fraction of the column pairs; 0 = only
+                                    after a rotation-free sweep.  Default 0.05: the rotation count falls from
+                                    about a third of the pairs to 1-2 % within one sweep near the FP32 limit, so
+                                    every fraction from 0.03 to 0.2 stops at that sweep, without a longer FP64
+                                    phase on the measured workloads.  The FP64 phase always ends on a
+                                    rotation-free sweep (the reference's rule). */
 } jsvd_mixed_config;
 
 #define JSVD_INNER_CROSS 0

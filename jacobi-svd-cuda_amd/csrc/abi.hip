@@ -583,7 +583,8 @@ namespace
             jsvd::throw_invalid("inner_order must be JSVD_INNER_CROSS or JSVD_INNER_FULL.");
         }
         if (mx.path == JSVD_PATH_MIXED && (mx.fp32_max_sweeps <= 0 || mx.orthogonalize_steps < 1 ||
-                                           mx.orthogonalize_steps > 8 || !(mx.fp32_tolerance < 1.0)))
+                                           mx.orthogonalize_steps > 8 || !(mx.fp32_tolerance < 1.0) ||
+                                           !(mx.fp32_stop_fraction >= 0.0 && mx.fp32_stop_fraction < 1.0)))
         {
             jsvd::throw_invalid("invalid mixed-precision configuration.");
         }

@@ -178,6 +178,12 @@ namespace jsvd
         JSVD_HIP_CHECK(hipGetLastError());
     }
 
+    void launch_sweep_continue(const int *rot, int stop, int *cont, hipStream_t stream)
+    {
+        hipLaunchKernelGGL(sweep_continue_kernel, dim3(1), dim3(64), 0, stream, rot, stop, cont);
+        JSVD_HIP_CHECK(hipGetLastError());
+    }
+
     void launch_iota(int *perm, int64_t n, hipStream_t stream)
     {
         hipLaunchKernelGGL(iota_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, perm, n);

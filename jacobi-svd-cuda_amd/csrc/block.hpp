@@ -105,6 +105,8 @@ namespace jsvd
     void launch_column_order(const double *norm2, int64_t n_pad, int *perm, const int *skip_if_zero,
                              hipStream_t stream);
     void launch_iota(int *perm, int64_t n, hipStream_t stream);
+    // *cont = (*rot > stop): the flag the next sweep's kernels exit on (0 = the phase has ended).
+    void launch_sweep_continue(const int *rot, int stop, int *cont, hipStream_t stream);
 
     void launch_convert_f64_to_f32(const double *src, float *dst, int64_t count, hipStream_t stream);
     void launch_convert_f32_to_f64(const float *src, double *dst, int64_t count, hipStream_t stream);

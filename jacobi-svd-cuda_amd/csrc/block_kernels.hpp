@@ -1239,6 +1239,16 @@ namespace jsvd
             }
         }
 
+        // cont = 1 while a phase goes on after the sweep whose rotation count is *rot: more than `stop` rotations
+        // (one thread; the kernels of the next sweep exit when cont is 0).
+        __global__ void sweep_continue_kernel(const int *__restrict__ rot, int stop, int *__restrict__ cont)
+        {
+            if (threadIdx.x == 0)
+            {
+                *cont = (*rot > stop) ? 1 : 0;
+            }
+        }
+
         __global__ void iota_kernel(int *__restrict__ perm, int64_t n)
         {
             const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -180,6 +180,7 @@ namespace jsvd
         JSVD_HIP_CHECK(hipMalloc(&perm_, sizeof(int) * ga_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&norm2_, sizeof(double) * ga_.n_pad));
         JSVD_HIP_CHECK(hipMalloc(&rot_, sizeof(int) * 2 * cap_));
+        JSVD_HIP_CHECK(hipMalloc(&cont_, sizeof(int) * 2 * cap_));
         JSVD_HIP_CHECK(hipMalloc(&off_, sizeof(unsigned long long) * 2 * cap_));
         JSVD_HIP_CHECK(hipHostMalloc(&h_rot_, sizeof(int) * 2 * cap_, hipHostMallocDefault));
         for (hipEvent_t &e : done_)
@@ -216,7 +217,7 @@ namespace jsvd
         for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
-                        static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(off_),
+                        static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(cont_), static_cast<void *>(off_),
                         static_cast<void *>(perm_), static_cast<void *>(norm2_), static_cast<void *>(racc_),
                         static_cast<void *>(dcol_), static_cast<void *>(chol_info_)})
         {
@@ -233,12 +234,14 @@ namespace jsvd
         }
     }
 
-    // Block sweeps with the same device-side early exit and lagged host polling as the plain path.
-    // Returns the executed sweep count (first rotation-free sweep included).
+    // Block sweeps with the same device-side early exit and lagged host polling as the plain path.  The phase
+    // ends after the first sweep with at most stop_rotations rotations (0: the first rotation-free sweep, the
+    // reference's rule); returns the executed sweep count, that sweep included.
     template <typename T>
     int BlockSolver::run_sweeps(const BlockGeometry &g, T *a, T *v, double eps, int max_sweeps, int *rot,
-                                unsigned long long *off, double &last_off)
+                                unsigned long long *off, double &last_off, int stop_rotations)
     {
+        int *cont = cont_ + (rot - rot_);
         JSVD_HIP_CHECK(hipMemsetAsync(rot, 0, sizeof(int) * max_sweeps, stream_));
         JSVD_HIP_CHECK(hipMemsetAsync(off, 0, sizeof(unsigned long long) * max_sweeps, stream_));
         int *h = h_rot_ + (rot - rot_);
@@ -252,7 +255,7 @@ namespace jsvd
         launch_iota(perm_, g.n_pad, stream_);
         for (int s = 0; s < max_sweeps; ++s)
         {
-            const int *skip = (s > 0) ? rot + (s - 1) : nullptr;
+            const int *skip = (s > 0) ? cont + (s - 1) : nullptr;
             if (s > 0)
             {
                 launch_column_order(norm2_, g.n_pad, perm_, skip, stream_);
@@ -264,13 +267,14 @@ namespace jsvd
                 launch_block_round<T>(g, a, v, r, eps, partial, q, flags_, rot + s, off + s, perm_, norm2_, skip,
                                       stream_);
             }
+            launch_sweep_continue(rot + s, stop_rotations, cont + s, stream_);
             JSVD_HIP_CHECK(hipMemcpyAsync(h + s, rot + s, sizeof(int), hipMemcpyDeviceToHost, stream_));
             JSVD_HIP_CHECK(hipEventRecord(done_[s & 1], stream_));
             enqueued = s + 1;
             if (s >= 1)
             {
                 JSVD_HIP_CHECK(hipEventSynchronize(done_[(s - 1) & 1]));
-                if (h[s - 1] == 0)
+                if (h[s - 1] <= stop_rotations)
                 {
                     break;
                 }
@@ -280,7 +284,7 @@ namespace jsvd
         int executed = enqueued;
         for (int s = 0; s < enqueued; ++s)
         {
-            if (h[s] == 0)
+            if (h[s] <= stop_rotations)
             {
                 executed = s + 1;
                 break;
@@ -365,7 +369,10 @@ namespace jsvd
         launch_convert_f64_to_f32(x64, a32_, x_elems, stream_);
         launch_identity_f32(v32_, g.ldv, g.n_pad, stream_);
         const int fp32_cap = std::max(1, mixed_.fp32_max_sweeps);
-        info.sweeps_fp32 = run_sweeps<float>(g, a32_, v32_, tol, fp32_cap, rot_, off_, info.offmax_fp32);
+        // the FP32 phase preconditions the FP64 one: it may end on a sweep that still rotates a few pairs
+        const double pairs = 0.5 * static_cast<double>(g.n) * static_cast<double>(g.n - 1);
+        const int stop = static_cast<int>(mixed_.fp32_stop_fraction * pairs);
+        info.sweeps_fp32 = run_sweeps<float>(g, a32_, v32_, tol, fp32_cap, rot_, off_, info.offmax_fp32, stop);
         launch_convert_f32_to_f64(v32_, v64_, v_elems, stream_);
         orthogonalize_v(v64_, w64_);
         launch_gemm_f64(false, false, g.lda, g.n_pad, g.n_pad, 1.0, x64, g.lda, v64_, g.ldv, 0.0, nullptr, 0, y64,

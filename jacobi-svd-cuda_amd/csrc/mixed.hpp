@@ -43,7 +43,7 @@ namespace jsvd
     private:
         template <typename T>
         int run_sweeps(const BlockGeometry &g, T *a, T *v, double eps, int max_sweeps, int *rot,
-                       unsigned long long *off, double &last_off);
+                       unsigned long long *off, double &last_off, int stop_rotations = 0);
         void orthogonalize_v(double *v, double *w);
         [[nodiscard]] bool qr_first() const { return mixed_.path == JSVD_PATH_MIXED && mixed_.qr_precondition != 0; }
         void set_geometry(int64_t m, int64_t n, const BlockGeometry &sweep);
@@ -79,6 +79,7 @@ namespace jsvd
         int *perm_ = nullptr;      // [n_pad] logical -> physical column of the current sweep
         double *norm2_ = nullptr;  // [n_pad] squared column norms after each column's last rotation
         int *rot_ = nullptr;                 // [2 * cap] rotation counts (FP32 sweeps, then FP64 sweeps)
+        int *cont_ = nullptr;                // [2 * cap] 1 while the phase goes on after sweep s
         unsigned long long *off_ = nullptr;  // [2 * cap]
         int *h_rot_ = nullptr;               // pinned mirror
         int cap_ = 0;

@@ -67,6 +67,7 @@ class MixedConfig(ctypes.Structure):
         ("block_width", ctypes.c_int32),
         ("qr_precondition", ctypes.c_int32),
         ("inner_order", ctypes.c_int32),
+        ("fp32_stop_fraction", ctypes.c_double),
     ]
 
 
@@ -91,7 +92,7 @@ class SolveInfo(ctypes.Structure):
 def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
                  fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None,
                  block_width: Optional[int] = None, qr_precondition: Optional[bool] = None,
-                 inner_order: Optional[int] = None) -> MixedConfig:
+                 inner_order: Optional[int] = None, fp32_stop_fraction: Optional[float] = None) -> MixedConfig:
     mx = MixedConfig()
     lib().jsvd_mixed_config_default(ctypes.byref(mx))
     mx.path = path
@@ -107,6 +108,8 @@ def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
         mx.fp32_tolerance = fp32_tolerance
     if orthogonalize_steps is not None:
         mx.orthogonalize_steps = orthogonalize_steps
+    if fp32_stop_fraction is not None:
+        mx.fp32_stop_fraction = fp32_stop_fraction
     return mx
 
 
