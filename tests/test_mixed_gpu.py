@@ -339,3 +339,32 @@ def test_inner_order_validated(torch_cuda):
     with pytest.raises(ValueError, match="inner_order"):
         jsvd.solve(a, jsvd.JacobiSvdConfig(epsilon=1e-14), path=jsvd.PATH_MIXED,
                    mixed=jsvd.mixed_config(jsvd.PATH_MIXED, inner_order=7))
+
+
+@pytest.mark.parametrize("fraction", [0.0, 0.05, 0.3])
+def test_fp32_phase_stop_fraction(torch_cuda, fraction):
+    """The FP32 phase ends on the first sweep that rotates at most fraction x pairs (0: a rotation-free sweep);
+    every earlier FP32 sweep rotated more; the FP64 phase still ends on a rotation-free sweep; accuracy holds."""
+    m, n = 1024, 256
+    rng = np.random.default_rng(21)
+    q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sig = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+    a = (q1 * sig) @ q2.T
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    plan = jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED,
+                     mixed=jsvd.mixed_config(jsvd.PATH_MIXED, fp32_stop_fraction=fraction))
+    got = plan.solve_host(a)
+    info = plan.last_info()
+    rot, _ = plan.sweep_stats()
+    k = info.sweeps_fp32
+    stop = int(fraction * n * (n - 1) / 2)
+    assert k >= 1 and len(rot) == k + info.sweeps_fp64
+    assert rot[k - 1] <= stop and all(r > stop for r in rot[:k - 1])
+    assert rot[-1] == 0 and info.converged == 1
+    s = sorted_desc(got.sigma)
+    assert np.max(np.abs(s - sig)) / sig[0] < 1e-12
+    assert rho_a(a, got.u, got.sigma, got.v) < 1e-13
+    assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
+    with pytest.raises(ValueError):
+        jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, fp32_stop_fraction=1.0))
