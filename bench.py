@@ -330,7 +330,7 @@ def batch_comparison(torch, jsvd, mats, tol, order, mixed_rate, shape, plain_sam
     }
 
 
-def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=4):
+def file_stream_leg(jsvd, a_host, cfg, sigma_mixed, records=12):
     """The tall workload as a *.mat file of `records` copies, solved by the streamed file path (mmap -> pinned
     double buffer -> HBM on a copy stream -> device decode, overlapping the next record's load with the current
     solve) on the mixed path, U / sigma / V written back to a *.mat file.  PCIe- and file-inclusive."""

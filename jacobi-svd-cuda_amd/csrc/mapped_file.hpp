@@ -140,6 +140,16 @@ namespace jsvd::mapped
             return dst;
         }
 
+        // Sizes the mapping for `bytes` in total up front (a writer that knows its output size): later appends up
+        // to that size neither remap nor flush the written prefix.  No effect when the capacity already suffices.
+        void reserve(std::size_t bytes)
+        {
+            if (bytes > capacity_)
+            {
+                remap(bytes);
+            }
+        }
+
         void append(const void *src, std::size_t bytes)
         {
             if (bytes == 0)

@@ -190,6 +190,18 @@ namespace jsvd
                 {
                     return;
                 }
+                if (out)
+                {
+                    // every output record's size is known from the input's record index: map the whole output
+                    // once (a growing mapping flushes its prefix to the disk at every doubling)
+                    std::size_t total = 0;
+                    for (const MatRecord &r : records_)
+                    {
+                        const std::size_t values = r.rows * r.columns + r.columns + r.columns * r.columns;
+                        total = mapped::add_checked(total, 3 * 2 * sizeof(std::uint64_t) + values * kValueBytes);
+                    }
+                    out->reserve(total);
+                }
                 const std::size_t in_bytes = max_count_ * kValueBytes;
                 PinnedBytes staging0(in_bytes), staging1(in_bytes);
                 DeviceBytes raw0(in_bytes), raw1(in_bytes), a0(in_bytes), a1(in_bytes);
