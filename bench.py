@@ -643,7 +643,7 @@ def main() -> None:
         from jsvd import serving as serving_mod
         log(f"[bench] serving: Poisson arrivals at {args.serving_rate}/s for {args.serving_seconds}s")
         pool = serving_mod.request_pool(96)
-        srv = jsvd.Server(lanes=4, large_path=jsvd.PATH_MIXED)
+        srv = jsvd.Server(lanes=8, large_path=jsvd.PATH_MIXED)
         try:
             serving = serving_mod.run_poisson(srv, pool, args.serving_rate, args.serving_seconds, check_every=10)
         finally:

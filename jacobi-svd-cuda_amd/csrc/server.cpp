@@ -433,7 +433,7 @@ void jsvd_server_config_default(jsvd_server_config *cfg)
         return;
     }
     std::memset(cfg, 0, sizeof(*cfg));
-    cfg->lanes = 4;
+    cfg->lanes = 8;
     cfg->small_max_rows = 256;
     cfg->small_max_columns = 128;
     cfg->max_batch = 4096;

@@ -142,7 +142,7 @@ def main(argv=None) -> None:
     ap.add_argument("--rates", default="500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=20.0)
     ap.add_argument("--pool", type=int, default=128)
-    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--large-path", default="mixed", choices=["plain", "mixed"])
     ap.add_argument("--check-every", type=int, default=25)
     ap.add_argument("--max-outstanding", type=int, default=20000,
