@@ -405,11 +405,13 @@ def host_description():
     return {"cpu_model": model, "nproc": usable, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
-def cpu_baseline(a_host, eps, max_sweeps, reps=3, c1_reps=5):
+def cpu_baseline(a_host, eps, max_sweeps, reps=3, c1_reps=5, c1_gpu_sigma=None):
     """The oracle (C restatement of the reference algorithm: same pair order, rotation, skip test and convergence
     rule; OpenMP over the disjoint pairs of a round, a barrier per round) timed on this host's cores: `reps` full
     solves of the workload (median), and the reference's own 512 x 256 case (one warm-up, median of `c1_reps`
-    full solves) with LAPACK dgesvj beside it on one thread."""
+    full solves) with LAPACK dgesvj beside it on one thread.  c1_gpu_sigma ({path name: sigma in the path's
+    column order}) are checked against both: the plain path column by column against the oracle (same column
+    order), every path sorted against dgesvj."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     host = host_description()
@@ -439,13 +441,21 @@ def cpu_baseline(a_host, eps, max_sweeps, reps=3, c1_reps=5):
     try:
         for _ in range(c1_reps):
             t0 = time.perf_counter()
-            oracle.dgesvj(c1)
+            _, svj_sigma, _, _ = oracle.dgesvj(c1)
             svj_times.append(time.perf_counter() - t0)
     finally:
         if limit is not None:
             limit.restore_original_limits()
     c1_med = float(np.median(c1_times))
     svj_med = float(np.median(svj_times))
+    checks = {}
+    svj_sorted = np.sort(svj_sigma)[::-1]
+    for name, sig in (c1_gpu_sigma or {}).items():
+        entry = {"sigma_relerr_vs_lapack_dgesvj": float(np.max(np.abs(np.sort(sig)[::-1] - svj_sorted) / svj_sorted))}
+        if name == "plain":
+            entry["sigma_relerr_vs_oracle_same_column_order"] = float(np.max(np.abs(sig - c1_res.sigma) /
+                                                                             np.abs(c1_res.sigma)))
+        checks[name] = entry
     return {
         "value": 1.0 / med,
         "unit": "solves/s",
@@ -460,6 +470,7 @@ def cpu_baseline(a_host, eps, max_sweeps, reps=3, c1_reps=5):
             "oracle_runs_s": c1_times, "threads": threads,
             "lapack_dgesvj_solves_per_s": 1.0 / svj_med, "lapack_dgesvj_median_s": svj_med,
             "lapack_dgesvj_threads": 1 if limit is not None else "library default",
+            "gpu_sigma_checks": checks,
         },
     }
 
@@ -615,6 +626,7 @@ def main() -> None:
         s_ref = np.sort(np.linalg.svd(c1, compute_uv=False))[::-1]
         s_jsv = lapack_dgejsv_sigma(c1)
         sig = {}
+        c1_raw_sigma = {"plain": c1_plain["sigma"].copy(), "mixed": c1_mixed["sigma"].copy()}
         for name, rep in (("plain", c1_plain), ("mixed", c1_mixed)):
             sig[name] = np.sort(rep.pop("sigma"))[::-1]
             rep["accuracy"]["sigma_relerr_vs_gesdd"] = float(np.max(np.abs(sig[name] - s_ref) / s_ref))
@@ -688,7 +700,8 @@ def main() -> None:
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline(a_host, eps, max_sweeps, reps=args.cpu_reps)
+        cpu = cpu_baseline(a_host, eps, max_sweeps, reps=args.cpu_reps,
+                           c1_gpu_sigma=c1_raw_sigma if reference_case is not None else None)
 
     for rep in (mixed, plain):
         if rep is not None:
