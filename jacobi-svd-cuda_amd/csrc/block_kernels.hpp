@@ -67,10 +67,11 @@ namespace jsvd
             return (KB == 64) ? kH / 2 : kH;
         }
 
-        // Loads rows [r0, r0+GH) of the 2 KB columns [X_I X_J] into LDS as xs[c][r] (column-major, padded).
+        // Loads rows [r0, r0+GH) of the 2 KB columns [X_I X_J] (physical columns cols[0 .. 2 KB), in LDS) into LDS
+        // as xs[c][r] (column-major, padded).
         template <typename T, int KB, int GH>
-        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, const int *__restrict__ perm, int I,
-                                         int J, int64_t r0, T (*xs)[GH + 1])
+        __device__ inline void load_slab(const T *__restrict__ x, int64_t ld, const int *cols, int64_t r0,
+                                         T (*xs)[GH + 1])
         {
             // 2 KB * GH elements, 256 threads; consecutive threads read consecutive rows of one column
             // (coalesced).  All loads are issued before the LDS stores so they are in flight together.
@@ -80,7 +81,7 @@ namespace jsvd
             for (int k = 0; k < kPer; ++k)
             {
                 const int e = threadIdx.x + 256 * k;
-                tmp[k] = x[pair_column<KB>(perm, I, J, e / GH) * ld + r0 + e % GH];
+                tmp[k] = x[static_cast<int64_t>(cols[e / GH]) * ld + r0 + e % GH];
             }
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
@@ -105,10 +106,7 @@ namespace jsvd
             constexpr int NT = KW / 16;         // tiles per dimension
             constexpr int TPW = NT * NT / 4;    // tiles per wave
             __shared__ T xs[KW][GH + 1];
-            if (skip_if_zero != nullptr && *skip_if_zero == 0)
-            {
-                return;
-            }
+            __shared__ int cols[KW]; // physical columns of the pair
             const int slot = blockIdx.x;
             const int ps = blockIdx.y;
             int I, J;
@@ -117,6 +115,23 @@ namespace jsvd
             {
                 return;
             }
+            // The column map and the skip flag are read together (one memory round trip before the slab loads,
+            // not two).
+            int col = 0;
+            if (threadIdx.x < KW)
+            {
+                col = static_cast<int>(pair_column<KB>(perm, I, J, threadIdx.x));
+            }
+            const int go = (skip_if_zero != nullptr) ? *skip_if_zero : 1;
+            if (threadIdx.x < KW)
+            {
+                cols[threadIdx.x] = col;
+            }
+            if (go == 0)
+            {
+                return;
+            }
+            __syncthreads();
             const int lane = threadIdx.x % 64;
             const int wave = threadIdx.x / 64;
             // tile k of this wave: row tile rt(k), column tile ct(k) (k is a compile-time index after unrolling,
@@ -140,7 +155,7 @@ namespace jsvd
                         __syncthreads(); // every wave is done with the previous rows
                     }
                     first = false;
-                    load_slab<T, KB, GH>(a, lda, perm, I, J, static_cast<int64_t>(s) * kH + h, xs);
+                    load_slab<T, KB, GH>(a, lda, cols, static_cast<int64_t>(s) * kH + h, xs);
                     __syncthreads();
                     for (int r0 = 0; r0 < GH; r0 += 4)
                     {
@@ -470,10 +485,9 @@ namespace jsvd
             __shared__ double red[kThreads / 64];
             __shared__ int any;
             __shared__ int needed;
-            if (skip_if_zero != nullptr && *skip_if_zero == 0)
-            {
-                return;
-            }
+            // The skip flag is read after the first partials' loads are issued (both in flight together: one
+            // memory round trip at the start, not two) and tested once the partials are summed.
+            int go = 1;
             const int slot = blockIdx.x;
             int I, J;
             block_pair(slot, round, Nb, I, J);
@@ -506,7 +520,26 @@ namespace jsvd
                 using V4 = dev::vec_t<T, 4>;
                 const int e0 = 4 * tid;
                 V4 acc = V4{0, 0, 0, 0};
-                int s = 0;
+                {
+                    // the first eight slabs (the last one repeated where there are fewer), then the skip flag
+                    V4 part[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        const int sk = min(k, slabs - 1);
+                        part[k] = *reinterpret_cast<const V4 *>(base + static_cast<int64_t>(sk) * (KW * KW) + e0);
+                    }
+                    go = (skip_if_zero != nullptr) ? *skip_if_zero : 1;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                    {
+                        if (k < slabs)
+                        {
+                            acc += part[k];
+                        }
+                    }
+                }
+                int s = 8;
                 for (; s + 8 <= slabs; s += 8)
                 {
                     V4 part[8];
@@ -538,6 +571,7 @@ namespace jsvd
             }
             else
             {
+                go = (skip_if_zero != nullptr) ? *skip_if_zero : 1;
                 for (int e = tid; e < KW * KW; e += kThreads)
                 {
                     const int i = e / KW;
@@ -568,6 +602,10 @@ namespace jsvd
                         g[0][L::at(i, j)] = acc;
                     }
                 }
+            }
+            if (go == 0)
+            {
+                return;
             }
             if (tid == 0)
             {
@@ -977,20 +1015,12 @@ namespace jsvd
             using V = typename Vec16<T>::type;
             __shared__ __align__(16) T xs[KW][kLd];
             __shared__ T qs[kQRegs ? 1 : KW][KW + 1];
-            if (skip_if_zero != nullptr && *skip_if_zero == 0)
-            {
-                return;
-            }
             const int slot = blockIdx.x;
             int I, J;
             block_pair(slot, round, Nb, I, J);
-            if (I >= nb || J >= nb || apply_flag[slot] == 0)
-            {
-                return;
-            }
             const int total = slabs_a + slabs_v;
             int s = blockIdx.y;
-            if (s >= total)
+            if (I >= nb || J >= nb || s >= total)
             {
                 return;
             }
@@ -1013,21 +1043,68 @@ namespace jsvd
                     }
                 }
             }
-            else
+            constexpr int kQPer = kQRegs ? 1 : KW * KW / 256; // Q entries per thread on their way to LDS
+            T qv[kQPer];
+            if constexpr (!kQRegs)
             {
-                for (int e = tid; e < KW * KW; e += 256)
+#pragma unroll
+                for (int k = 0; k < kQPer; ++k)
                 {
-                    qs[e / KW][e % KW] = qg[e];
+                    qv[k] = qg[tid + 256 * k];
                 }
             }
             // This thread's vectors of a slab: column (tid + 256 k) / kColVecs, rows ((tid + 256 k) % kColVecs) * kVec
             // .. + kVec; the row offset is the same for every k (256 is a multiple of kColVecs).
             static_assert(256 % kColVecs == 0, "apply load layout");
+            static_assert(kQRegs || (KW * KW) % 256 == 0, "apply Q layout");
             int col_of[kPer];
 #pragma unroll
             for (int k = 0; k < kPer; ++k)
             {
                 col_of[k] = static_cast<int>(pair_column<KB>(perm, I, J, (tid + 256 * k) / kColVecs));
+            }
+            // Q, the column map, the skip flag and the pair's rotation flag are read together: one memory round
+            // trip before the slab loads, not three.  The flags are loaded unconditionally (with no skip flag: the
+            // rotation flag twice), so that neither load has a block of its own to wait in; the empty asm keeps
+            // the vector loads ahead of the test.
+            const int flag_word = *((skip_if_zero != nullptr) ? skip_if_zero : apply_flag + slot);
+            const int rotated = apply_flag[slot];
+            if constexpr (kQRegs)
+            {
+#pragma unroll
+                for (int kk = 0; kk < KW / 4; ++kk)
+                {
+#pragma unroll
+                    for (int t = 0; t < CT; ++t)
+                    {
+                        dev::keep(bq[kk][t]);
+                    }
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < kQPer; ++k)
+                {
+                    dev::keep(qv[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                dev::keep(col_of[k]);
+            }
+            if (flag_word == 0 || rotated == 0)
+            {
+                return;
+            }
+            if constexpr (!kQRegs)
+            {
+#pragma unroll
+                for (int k = 0; k < kQPer; ++k)
+                {
+                    qs[(tid + 256 * k) / KW][(tid + 256 * k) % KW] = qv[k];
+                }
             }
             const int vrow = (tid % kColVecs) * kVec;
             V pre[kPer];

@@ -72,6 +72,13 @@ namespace jsvd::plain
         return z;
     }
 
+    // A loaded chunk kept ahead of a later branch (dev::keep on both halves).
+    __device__ inline void keep2(double2 &v)
+    {
+        dev::keep(v.x);
+        dev::keep(v.y);
+    }
+
     // Chunk idx of a column of len2 double2 chunks (odd_tail: the .y of the last one is padding), as zero where
     // idx >= len2 or !use.  The load itself is unconditional -- chunk 0 of the column where the chunk is not
     // used -- so a run of these issues every load before the first wait: a bounds branch around each load made
@@ -125,7 +132,7 @@ namespace jsvd::plain
         // the pair's slot of the rotation counter and the cosine maximum (count_slots is a power of two):
         // spreading the one atomic per workgroup over slots keeps same-address atomics from serialising
         const int cslot = static_cast<int>(blockIdx.x) & (count_slots - 1);
-        if (skip_if_zero != nullptr && *skip_if_zero == 0)
+        if (EA == 0 && skip_if_zero != nullptr && *skip_if_zero == 0)
         {
             return;
         }
@@ -167,6 +174,9 @@ namespace jsvd::plain
                 xv[k] = raw_chunk(vp, idx, n2, true);
                 yv[k] = raw_chunk(vq, idx, n2, true);
             }
+            // The skip flag is read after the column loads are issued, so both are in flight together (one memory
+            // round trip at the start of a workgroup, not two), and tested once the loads have landed.
+            const int go = (skip_if_zero != nullptr) ? *skip_if_zero : 1;
 #pragma unroll
             for (int k = 0; k < EA; ++k)
             {
@@ -180,6 +190,23 @@ namespace jsvd::plain
                 const int64_t idx = static_cast<int64_t>(k) * T + tid;
                 xv[k] = fix_chunk(xv[k], idx, n2, odd_tail_v, true);
                 yv[k] = fix_chunk(yv[k], idx, n2, odd_tail_v, true);
+            }
+            // (the empty asm keeps the loads ahead of the test instead of sunk past it)
+#pragma unroll
+            for (int k = 0; k < EA; ++k)
+            {
+                keep2(ap[k]);
+                keep2(aq[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < EV; ++k)
+            {
+                keep2(xv[k]);
+                keep2(yv[k]);
+            }
+            if (go == 0)
+            {
+                return;
             }
 #pragma unroll
             for (int k = 0; k < EA; ++k)
@@ -382,10 +409,6 @@ namespace jsvd::plain
         constexpr int NB = (2 * W * CC * 64 * 16 <= 160 * 1024) ? 2 : 1;
         static_assert(W * CC * 64 * 16 <= 160 * 1024, "round span stage exceeds LDS");
         __shared__ double2 stage[NB][W][CC][64];
-        if (skip_if_zero != nullptr && *skip_if_zero == 0)
-        {
-            return;
-        }
         const int H = N / 2;
         int a, b;
         span_owned(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), H, own, own_end, a, b);
@@ -409,7 +432,7 @@ namespace jsvd::plain
             return c < CA ? static_cast<int64_t>(c % EA) * TT + (c / EA) * 64 + lane
                           : static_cast<int64_t>(c - CA) * 64 + lane;
         };
-        auto load2 = [&](int col_t, int col_b, double2 (&x)[CC], double2 (&y)[CC]) {
+        auto load_raw2 = [&](int col_t, int col_b, double2 (&x)[CC], double2 (&y)[CC]) {
             const bool ut = col_t < n, ub = col_b < n;
             const double2 *at = reinterpret_cast<const double2 *>(a_in + static_cast<int64_t>(ut ? col_t : 0) * lda);
             const double2 *ab = reinterpret_cast<const double2 *>(a_in + static_cast<int64_t>(ub ? col_b : 0) * lda);
@@ -422,6 +445,9 @@ namespace jsvd::plain
                 x[c] = c < CA ? raw_chunk(at, idx, m2, ut) : raw_chunk(vt, idx, n2, ut);
                 y[c] = c < CA ? raw_chunk(ab, idx, m2, ub) : raw_chunk(vb, idx, n2, ub);
             }
+        };
+        auto fix2 = [&](int col_t, int col_b, double2 (&x)[CC], double2 (&y)[CC]) {
+            const bool ut = col_t < n, ub = col_b < n;
 #pragma unroll
             for (int c = 0; c < CC; ++c)
             {
@@ -461,7 +487,14 @@ namespace jsvd::plain
         int q = dev::circle_at(N - 1 - k, r0, N);
         if (in_cone)
         {
-            load2(p, q, tp, bt);
+            load_raw2(p, q, tp, bt);
+        }
+        // the skip flag is read after the first column loads are issued (both in flight together) and tested once
+        // they have landed
+        const int go = (skip_if_zero != nullptr) ? *skip_if_zero : 1;
+        if (in_cone)
+        {
+            fix2(p, q, tp, bt);
         }
         else
         {
@@ -471,6 +504,16 @@ namespace jsvd::plain
                 tp[c] = zero2();
                 bt[c] = zero2();
             }
+        }
+#pragma unroll
+        for (int c = 0; c < CC; ++c)
+        {
+            keep2(tp[c]); // the loads stay ahead of the skip test
+            keep2(bt[c]);
+        }
+        if (go == 0)
+        {
+            return;
         }
         int my_rot = 0;
         double my_cos = 0.0;
