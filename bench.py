@@ -661,7 +661,7 @@ def main() -> None:
         finally:
             srv.close()
         serving["workload"] = ("serving mix: m = exp U(ln16, ln2048), n ~ U{4..min(m,512)}, kappa 10^U(1,10), "
-                               "relerr 1e-6 / 1e-12 alternating, 96 distinct matrices cycled, 4 lanes, HBM "
+                               "relerr 1e-6 / 1e-12 alternating, 96 distinct matrices cycled, 8 lanes, HBM "
                                "budget 64 GiB, large requests on the mixed path")
         log(f"[bench] serving: {serving}")
 
