@@ -4,11 +4,14 @@
 //   A (row-major) -> A64 column-major, padded (LDS-tiled transpose)
 //   QR: d_j = |a_j|, B = A D^-1; three Cholesky QR passes, the first with a diagonal shift:
 //       G = B^T B (+ s I), R_k = chol(G), B <- B R_k^-1; Q = B, R = R_3 R_2 R_1 D   (A = Q R)
-//   M = R^T (n x n); M32 = fp32(M), V32 = I; FP32 block sweeps to the FP32 tolerance (MFMA, FP32 accumulation)
+//   M = R^T (n x n); with qr_precondition 2 a second QR, M = Q2 R2 by the same column-scaled shifted Cholesky
+//       QR, and the sweeps run on N = R2^T instead of M (an LQ step of R: N N^T = R2^T R2 = M^T M is the same
+//       Gram matrix, and N's columns are closer to orthogonal; otherwise N below stands for M)
+//   N32 = fp32(N), V32 = I; FP32 block sweeps to the FP32 tolerance (MFMA, FP32 accumulation)
 //   V64 = fp64(V32); Newton-Schulz V <- V (3I - V^T V) / 2 (FP64 GEMMs) until V is orthogonal in FP64
-//   X = M V64 (FP64 GEMM); FP64 block sweeps on (X, V64) to epsilon
-//   sigma_j = |x_j|, V_A = X / sigma (row-major), U_A = Q V64 written row-major by one GEMM, columns with
-//   sigma_j <= epsilon zeroed as the reference zeroes them
+//   X = N V64 (FP64 GEMM); FP64 block sweeps on (X, V64) to epsilon
+//   sigma_j = |x_j|, Y = X / sigma; with the second QR U_A = Q Y and V_A = Q2 V64, with one QR V_A = Y and
+//   U_A = Q V64 (row-major, one GEMM each); U columns with sigma_j <= epsilon zeroed as the reference zeroes them
 // The products (B^T B and V^T V on their upper triangle only, R_k R_acc on triangular operands) and the triangular
 // solves are this library's FP64 MFMA kernels (dense.hip), as are the tiled Cholesky factorisations
 // (cholesky.hip).  The shifted Cholesky QR follows Fukaya, Kannan, Nakatsukasa,
@@ -194,6 +197,15 @@ namespace jsvd
         {
             JSVD_HIP_CHECK(hipMalloc(&a64b_, sizeof(double) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&w64_, sizeof(double) * v_elems));
+            if (qr_first() && mixed_.qr_precondition == 2)
+            {
+                // the second QR's Q2 and N = R2^T, in the sweep geometry (ldv x n_pad covers it: ldv >= lda of an
+                // n x n sweep, both multiples of the slab height)
+                JSVD_HIP_CHECK(hipMalloc(&q2_, sizeof(double) * v_elems));
+                JSVD_HIP_CHECK(hipMalloc(&n2_, sizeof(double) * v_elems));
+                JSVD_HIP_CHECK(hipMemsetAsync(q2_, 0, sizeof(double) * v_elems, stream_));
+                JSVD_HIP_CHECK(hipMemsetAsync(n2_, 0, sizeof(double) * v_elems, stream_));
+            }
             // n_pad x n_pad products and the Cholesky factors, padded to the factorisation's 64-row tiles
             const int64_t n64 = cholesky_padded_size(ga_.n_pad);
             JSVD_HIP_CHECK(hipMalloc(&gram64_, sizeof(double) * n64 * n64));
@@ -214,7 +226,7 @@ namespace jsvd
 
     BlockSolver::~BlockSolver()
     {
-        for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
+        for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(q2_), static_cast<void *>(n2_), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
                         static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(cont_), static_cast<void *>(off_),
@@ -379,14 +391,11 @@ namespace jsvd
                         g.lda, 0, dws_, stream_);
     }
 
-    bool BlockSolver::factor_qr()
+    bool BlockSolver::factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out)
     {
-        const int64_t m = ga_.m;
-        const int64_t n = ga_.n;
-        const int64_t lda = ga_.lda;
-        const int64_t ldr = cholesky_padded_size(ga_.n_pad);
-        hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(ga_.n_pad)), dim3(256), 0, stream_, a64_,
-                           ga_.lda, ga_.m, ga_.n, dcol_);
+        const int64_t ldr = cholesky_padded_size(g_.n_pad);
+        hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(g_.n_pad)), dim3(256), 0, stream_, x,
+                           lda, m, n, dcol_);
         JSVD_HIP_CHECK(hipGetLastError());
         // |B|_2^2 <= |B|_F^2 <= n for unit columns
         const double u = std::ldexp(1.0, -53);
@@ -394,7 +403,7 @@ namespace jsvd
         for (int pass = 0; pass < 3; ++pass)
         {
             // G = B^T B on the upper triangle (the only part the factorisation reads)
-            launch_gemm_f64(true, false, n, n, m, 1.0, a64_, lda, a64_, lda, 0.0, nullptr, 0, gram64_, ldr,
+            launch_gemm_f64(true, false, n, n, m, 1.0, x, lda, x, lda, 0.0, nullptr, 0, gram64_, ldr,
                             dense::kUpperOut, dws_, stream_);
             if (pass == 0)
             {
@@ -404,7 +413,7 @@ namespace jsvd
             }
             launch_cholesky_upper(gram64_, ldr, n, chol_info_ + pass, stream_);
             // B <- B R^-1 (the factor's identity padding up to ldr is the padding the solve needs)
-            launch_trsm_right_upper_f64(a64_, lda, m, n, gram64_, ldr, true, nullptr, dinv_, dws_, stream_);
+            launch_trsm_right_upper_f64(x, lda, m, n, gram64_, ldr, true, nullptr, dinv_, dws_, stream_);
             if (pass == 0)
             {
                 hipLaunchKernelGGL(copy_upper_kernel, dim3(grid_of(static_cast<int64_t>(n) * n)), dim3(256), 0,
@@ -426,11 +435,11 @@ namespace jsvd
         JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
         if (status[0] != 0 || status[1] != 0 || status[2] != 0)
         {
-            return false; // a factor was not positive definite: the caller solves A directly
+            return false; // a factor was not positive definite
         }
-        // M = (R D)^T into a64b_ with the sweep geometry's padding
+        // (R D)^T into m_out with the sweep geometry's padding
         hipLaunchKernelGGL(transpose_scaled_kernel, dim3(grid_of(g_.lda * g_.n_pad)), dim3(256), 0, stream_, racc_,
-                           static_cast<int64_t>(ldr), dcol_, static_cast<int64_t>(n), a64b_, g_.lda, g_.n_pad);
+                           static_cast<int64_t>(ldr), dcol_, static_cast<int64_t>(n), m_out, g_.lda, g_.n_pad);
         JSVD_HIP_CHECK(hipGetLastError());
         return true;
     }
@@ -439,21 +448,47 @@ namespace jsvd
                                      jsvd_solve_info &info)
     {
         launch_transpose(d_a, ga_.n, a64_, ga_.lda, ga_.m, ga_.n, stream_);
-        if (!factor_qr())
+        // A = Q R: Q in a64_, M = R^T in a64b_
+        if (!factor_qr(a64_, ga_.m, ga_.n, ga_.lda, a64b_))
         {
-            return false;
+            return false; // the caller solves A directly
         }
-        // sweeps on M = R^T (n x n): FP32 phase, then X = M V, FP64 phase
+        // M = Q2 R2 on a copy of M: Q2 in q2_, N = R2^T in n2_; the sweeps run on N, or on M when a factor of the
+        // second QR breaks down
+        const double *sweep_in = a64b_;
+        bool second = false;
+        if (mixed_.qr_precondition == 2 && q2_ != nullptr)
+        {
+            JSVD_HIP_CHECK(hipMemcpyAsync(q2_, a64b_, sizeof(double) * g_.lda * g_.n_pad, hipMemcpyDeviceToDevice,
+                                          stream_));
+            second = factor_qr(q2_, g_.n, g_.n, g_.lda, n2_);
+            sweep_in = second ? n2_ : a64b_;
+        }
+        // FP32 phase, then X = N V (or M V), FP64 phase
         const double tol = (fp32_tol_requested_ > 0.0) ? fp32_tol_requested_ : default_fp32_tolerance(g_.m);
-        fp32_phase(g_, a64b_, w64_, tol, info);
+        fp32_phase(g_, sweep_in, w64_, tol, info);
         info.sweeps_fp64 = run_sweeps<double>(g_, w64_, v64_, cfg_.epsilon, cfg_.max_sweeps, rot_ + cap_, off_ + cap_,
                                               info.offmax_fp64);
         info.converged = (h_rot_[cap_ + info.sweeps_fp64 - 1] == 0) ? 1 : 0;
-        // sigma_j = |x_j| and V_A = X / sigma (row-major), no cutoff: a column is normalised whenever sigma_j > 0
+        // sigma_j = |x_j| and Y = X / sigma (row-major into d_v), no cutoff: a column is normalised whenever
+        // sigma_j > 0
         launch_u_sigma(w64_, g_.lda, g_.n, g_.n, 0.0, d_v, d_sigma, stream_);
-        // U_A = Q V_M, produced row-major directly: U^T = V_M^T Q^T is column-major n x m with leading dimension n
-        launch_gemm_f64(false, false, ga_.m, ga_.n, ga_.n, 1.0, a64_, ga_.lda, v64_, g_.ldv, 0.0, nullptr, 0, d_u,
-                        ga_.n, dense::kRowMajorCD, dws_, stream_);
+        if (second)
+        {
+            // A = Q M^T = Q N Q2^T = Q Y Sigma (Q2 V)^T: U_A = Q Y (Y row-major in d_v is Y^T column-major with
+            // leading dimension n), then V_A = Q2 V over it, both produced row-major
+            launch_gemm_f64(false, true, ga_.m, ga_.n, ga_.n, 1.0, a64_, ga_.lda, d_v, ga_.n, 0.0, nullptr, 0, d_u,
+                            ga_.n, dense::kRowMajorCD, dws_, stream_);
+            launch_gemm_f64(false, false, ga_.n, ga_.n, ga_.n, 1.0, q2_, g_.lda, v64_, g_.ldv, 0.0, nullptr, 0, d_v,
+                            ga_.n, dense::kRowMajorCD, dws_, stream_);
+        }
+        else
+        {
+            // V_A = Y; U_A = Q V_M, produced row-major directly: U^T = V_M^T Q^T is column-major n x m with leading
+            // dimension n
+            launch_gemm_f64(false, false, ga_.m, ga_.n, ga_.n, 1.0, a64_, ga_.lda, v64_, g_.ldv, 0.0, nullptr, 0, d_u,
+                            ga_.n, dense::kRowMajorCD, dws_, stream_);
+        }
         hipLaunchKernelGGL(mask_columns_kernel, dim3(grid_of(ga_.m * ga_.n)), dim3(256), 0, stream_, d_u, ga_.m,
                            ga_.n, static_cast<const double *>(d_sigma), cfg_.epsilon);
         JSVD_HIP_CHECK(hipGetLastError());

@@ -15,10 +15,12 @@ namespace jsvd
     // tolerance, the FP32 V re-orthogonalised in FP64 (Newton-Schulz), M V formed in FP64, FP64 block sweeps to
     // the caller's epsilon; then sigma_j = |x_j|, V_A = X / sigma (X = the converged M V) and U_A = Q V_M.  Sweeping
     // R^T instead of A diagonalises R R^T, one QR-algorithm step further along than A^T A, which cuts the sweep
-    // count of both phases; the sweeps also run on n instead of m rows.  When a Cholesky factor breaks down or a
-    // sigma comes out exactly zero (V_A cannot be normalised) the solve is redone on A directly (FP32 sweeps,
-    // Newton-Schulz, A V, FP64 sweeps) with the reference's output conventions.  qr_precondition = 0 selects that
-    // direct form outright.
+    // count of both phases; the sweeps also run on n instead of m rows.  qr_precondition = 2 adds a second QR,
+    // M = Q2 R2 (the same column-scaled shifted Cholesky QR), and sweeps N = R2^T instead (one step further again,
+    // the LQ step of preconditioned Jacobi); then Y = X / sigma, U_A = Q Y and V_A = Q2 V.  When a factor of the
+    // second QR breaks down the solve goes on with one QR; when one of the first breaks down or a sigma comes out
+    // exactly zero (V_A cannot be normalised) the solve is redone on A directly (FP32 sweeps, Newton-Schulz, A V,
+    // FP64 sweeps) with the reference's output conventions.  qr_precondition = 0 selects that direct form outright.
     class BlockSolver
     {
     public:
@@ -49,9 +51,9 @@ namespace jsvd
         void set_geometry(int64_t m, int64_t n, const BlockGeometry &sweep);
         // FP32 sweeps on (x32 = fp32(x64), V32 = I) with geometry g, V64 <- orthogonalised fp64(V32), y = x64 V64.
         void fp32_phase(const BlockGeometry &g, const double *x64, double *y64, double tol, jsvd_solve_info &info);
-        // Column-scaled shifted Cholesky QR of a64_ (m x n): a64_ <- Q, M = R^T into a64b_ (sweep geometry).
-        // False when a Cholesky factor breaks down.
-        bool factor_qr();
+        // Column-scaled shifted Cholesky QR of x (m x n, leading dimension lda): x <- Q, (R D)^T into m_out (sweep
+        // geometry).  False when a Cholesky factor breaks down.
+        bool factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out);
         void solve_direct(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
         bool solve_qr_first(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
 
@@ -64,6 +66,8 @@ namespace jsvd
         double *dinv_ = nullptr;     // inverted diagonal blocks of the triangular solve
         double *a64_ = nullptr;
         double *a64b_ = nullptr;
+        double *q2_ = nullptr;       // second QR of the mixed path: Q2 (n x n, sweep geometry), or null
+        double *n2_ = nullptr;       // second QR: N = (R2 D2)^T, the matrix the sweeps run on
         double *v64_ = nullptr;
         double *w64_ = nullptr;
         double *gram64_ = nullptr;

@@ -91,7 +91,7 @@ class SolveInfo(ctypes.Structure):
 
 def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
                  fp32_tolerance: Optional[float] = None, orthogonalize_steps: Optional[int] = None,
-                 block_width: Optional[int] = None, qr_precondition: Optional[bool] = None,
+                 block_width: Optional[int] = None, qr_precondition: Optional[int] = None,
                  inner_order: Optional[int] = None, fp32_stop_fraction: Optional[float] = None) -> MixedConfig:
     mx = MixedConfig()
     lib().jsvd_mixed_config_default(ctypes.byref(mx))
@@ -99,7 +99,8 @@ def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
     if inner_order is not None:
         mx.inner_order = inner_order
     if qr_precondition is not None:
-        mx.qr_precondition = int(bool(qr_precondition))
+        # 0 = sweep A, 1 = one QR (sweep R^T), 2 = two QRs (sweep R2^T); a bool selects 0 or 1
+        mx.qr_precondition = int(qr_precondition)
     if block_width is not None:
         mx.block_width = block_width
     if fp32_max_sweeps is not None:

@@ -368,3 +368,36 @@ def test_fp32_phase_stop_fraction(torch_cuda, fraction):
     assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11
     with pytest.raises(ValueError):
         jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, fp32_stop_fraction=1.0))
+
+
+@pytest.mark.parametrize("m,n,kind", [(1024, 256, "geometric"), (600, 200, "gaussian"), (256, 256, "geometric")])
+def test_mixed_second_qr(torch_cuda, m, n, kind):
+    """qr_precondition 2 (A = Q R, then R^T = Q2 R2, sweeps on R2^T, U = Q Y, V = Q2 V_N) against one QR: the same
+    sorted sigma (1e-12 normwise; 1e-12 of the exact sigma of the geometric construction), U and V orthogonal,
+    A = U S V^T, converged; on the kappa = 1e8 construction the FP32 phase needs no more sweeps than with one QR."""
+    rng = np.random.default_rng(13)
+    if kind == "geometric":
+        q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+        q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        exact = 10.0 ** (-8.0 * np.arange(n) / (n - 1))
+        a = (q1 * exact) @ q2.T
+    else:
+        exact = None
+        a = rng.standard_normal((m, n))
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    out = {}
+    for qr in (1, 2):
+        print(f"second qr: {m}x{n} {kind} qr_precondition={qr}", flush=True)
+        got, info = jsvd.solve(a, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=qr))
+        assert info.converged == 1, qr
+        assert rho_a(a, got.u, got.sigma, got.v) < 1e-13, qr
+        assert rho_orth(got.u) < 1e-11 and rho_orth(got.v) < 1e-11, qr
+        out[qr] = (sorted_desc(got.sigma), info)
+    s1, i1 = out[1]
+    s2, i2 = out[2]
+    assert np.max(np.abs(s2 - s1)) / s1[0] < 1e-12
+    if exact is not None:
+        assert np.max(np.abs(s2 - exact)) / exact[0] < 1e-12
+        assert i2.sweeps_fp32 <= i1.sweeps_fp32, (i1.as_dict(), i2.as_dict())
+    with pytest.raises(ValueError, match="qr_precondition"):
+        jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=3))
