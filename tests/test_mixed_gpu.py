@@ -230,13 +230,16 @@ def test_block_width_argument_checks(torch_cuda):
     assert np.max(np.abs(sorted_desc(got.sigma) - s_ref)) / s_ref[0] < 1e-12
 
 
-@pytest.mark.parametrize("path", [jsvd.PATH_PLAIN_F64, jsvd.PATH_BLOCK_F64, jsvd.PATH_MIXED])
-def test_plan_reshape_equals_fresh_plans(torch_cuda, path):
+@pytest.mark.parametrize("path,qr", [(jsvd.PATH_PLAIN_F64, 1), (jsvd.PATH_BLOCK_F64, 1), (jsvd.PATH_MIXED, 1),
+                                     (jsvd.PATH_MIXED, 2)])
+def test_plan_reshape_equals_fresh_plans(torch_cuda, path, qr):
     """One plan re-targeted at several smaller shapes and epsilons (jsvd_plan_reshape) gives, bit for bit, the
-    results of fresh plans created for each shape; shapes beyond the capacity are rejected."""
+    results of fresh plans created for each shape (mixed: with one QR and with the second QR); shapes beyond the
+    capacity are rejected."""
     import torch
     cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
-    big = jsvd.Plan(400, 160, cfg, path=path)
+    mx = jsvd.mixed_config(path, qr_precondition=qr)
+    big = jsvd.Plan(400, 160, cfg, path=path, mixed=mx)
     shapes = [(400, 160), (257, 97), (300, 40), (101, 101), (399, 159)]
     if path == jsvd.PATH_PLAIN_F64:
         shapes.append((50, 7))
@@ -246,7 +249,7 @@ def test_plan_reshape_equals_fresh_plans(torch_cuda, path):
         big.reshape(m, n, eps)
         d_a = torch.from_numpy(a).cuda()
         outs = []
-        for plan in (big, jsvd.Plan(m, n, jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=60), path=path)):
+        for plan in (big, jsvd.Plan(m, n, jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=60), path=path, mixed=mx)):
             d_u = torch.empty((m, n), dtype=torch.float64, device="cuda")
             d_s = torch.empty(n, dtype=torch.float64, device="cuda")
             d_v = torch.empty((n, n), dtype=torch.float64, device="cuda")
