@@ -9,6 +9,8 @@ scratch slices or the buffers, stops the run with an error.
   255x127 items, and a batch with an item beyond the limits), under every launch shape of the kernel.
 * Block kernels: the sweep loop of the block and mixed paths (column ordering, intra-block step, Gram / inner
   sweep / apply rounds) at width 32 in both inner orderings, FP64 and FP32, against LAPACK's singular values.
+* Column ordering: the bitonic network gives the permutation of numpy's stable sort by descending norm, ties by
+  column index.
 
 The model's own checks are pinned by tests/simt_model/selftest.cpp (planted faults).
 """
@@ -42,8 +44,8 @@ def _build(name):
 @pytest.fixture(scope="module")
 def model_bins():
     with cf.ThreadPoolExecutor(3) as ex:
-        batch, selftest, block = ex.map(_build, ["batch_model", "selftest", "block_model"])
-    return batch, selftest, block
+        batch, selftest, block, order = ex.map(_build, ["batch_model", "selftest", "block_model", "order_model"])
+    return batch, selftest, block, order
 
 
 def _run_model(binary, mats, prec, tol, threads, lds, max_sweeps, max_rows=256, max_cols=128):
@@ -182,3 +184,24 @@ def test_block_sweeps_on_model(model_bins):
             else:
                 assert np.max(np.abs(s - ref)) / ref[0] < 1e-5, order
                 assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-4, order
+
+
+@pytest.mark.parametrize("n_pad", [1, 96, 1000, 1024, 2080, 4096])
+def test_column_order_on_model(model_bins, n_pad):
+    """column_order_kernel's permutation (bitonic network over the next power of two, padded with keys below every
+    norm) equals a stable sort by descending squared norm with ties by column index, with repeated keys, zero norms
+    (padding columns) and signed zeros among the keys, for powers of two and between them."""
+    order = model_bins[3]
+    rng = np.random.default_rng(n_pad)
+    keys = rng.choice([0.0, -0.0, 1.0, 2.5, 1e-300], size=n_pad) * (rng.random(n_pad) < 0.3)
+    keys = np.where(rng.random(n_pad) < 0.5, rng.random(n_pad) * 10.0, keys)
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in"), os.path.join(d, "out")
+        with open(fi, "wb") as f:
+            np.array([n_pad], np.int64).tofile(f)
+            keys.astype(np.float64).tofile(f)
+        r = subprocess.run([order, fi, fo], capture_output=True, text=True, env=ENV)
+        assert r.returncode == 0, r.stderr[-4000:]
+        perm = np.fromfile(fo, np.int32)
+    expected = np.lexsort((np.arange(n_pad), -keys))
+    assert np.array_equal(perm, expected)
