@@ -1,7 +1,7 @@
 /*
  * jacobi_oracle.c -- CPU restatement of the reference's cyclic one-sided (Hestenes) Jacobi SVD.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP solver: it is loaded by tests/,
+ * TEST INFRASTRUCTURE ONLY.  The HIP solver's results are checked against this file: it is loaded by tests/,
  * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, and by nothing else.  The
  * product library (libjsvd.so) never links or calls it.
  *

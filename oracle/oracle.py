@@ -1,7 +1,7 @@
 """ctypes front-end of the CPU oracle (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py import this module, and
-only as the checker.  The product library never calls it.
+only to check results against.  The product library never calls it.
 
 Two oracles live here:
 

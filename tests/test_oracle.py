@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/jacobi_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/jacobi_oracle.c) before results are checked against it.
 
 The reference ships no tests or golden vectors, so the oracle is pinned by:
   * hand-derived known answers (round-robin schedules n=4, n=5; 2x2 SVD) in tests/golden/closed_form.json,
