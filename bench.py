@@ -577,13 +577,17 @@ def main() -> None:
     if args.roofline_only:
         # The kernels whose HBM traffic the committed PMC summary reports (profiles/traffic.md): the roofline
         # launches, one mixed and one plain solve of the workload (Gram, inner step, apply, Cholesky, transposes,
-        # round kernel), a 2,000-matrix batch under every launch shape, and a 256 MiB copy by the library's copy
-        # kernel as the counters' calibration (known bytes read and written).
+        # round kernel), one plain solve of the 512 x 256 case (round spans), a 2,000-matrix batch under every
+        # launch shape, and a 256 MiB copy by the library's copy kernel as the counters' calibration (known bytes
+        # read and written).
         eig = eigen_kernel_roofline(torch, jsvd)
         avg_us, bytes_per_launch, _, _ = round_kernel_roofline(torch, jsvd, a_host, eps)
         time_path(torch, jsvd, jsvd.PATH_MIXED, a_host, sigma_true, cfg, 1, 0)
         time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, a_host, None,
                   jsvd.JacobiSvdConfig(epsilon=eps, max_sweeps=2), 1, 0)   # two sweeps: every kernel of the path
+        c1, _, c1_eps, c1_sweeps, _ = make_workload("plain")
+        time_path(torch, jsvd, jsvd.PATH_PLAIN_F64, c1, None,
+                  jsvd.JacobiSvdConfig(epsilon=c1_eps, max_sweeps=c1_sweeps), 1, 0)
         batch_leg(torch, jsvd, 2000, True, repeats=1, checked=0)   # every launch shape, then the tuned one
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")   # 256 MiB
         dst = torch.empty_like(src)
