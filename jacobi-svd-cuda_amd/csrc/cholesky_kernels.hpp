@@ -85,8 +85,14 @@ namespace jsvd
                             first_bad = (first_bad == kT) ? pc : first_bad;
                             piv = 1.0; // keep the arithmetic finite; the factorisation is reported as failed
                         }
-                        const double r = sqrt(piv);
-                        const double u = x[c] / r; // R[pc][lane] for lane > pc
+                        // 1 / sqrt(piv): the hardware estimate and two Newton steps y (3 - piv y^2) / 2 (full FP64
+                        // precision), then r = piv / sqrt(piv) and the pivot row scaled by a product: a shorter
+                        // dependent chain per pivot than a square root and a division
+                        double inv = __builtin_amdgcn_rsq(piv);
+                        inv = inv * fma(-0.5 * piv * inv, inv, 1.5);
+                        inv = inv * fma(-0.5 * piv * inv, inv, 1.5);
+                        const double r = piv * inv;
+                        const double u = x[c] * inv; // R[pc][lane] for lane > pc
                         x[c] = (lane == pc) ? r : u;
 #pragma unroll
                         for (int i = c + 1; i < kSub; ++i)

@@ -83,7 +83,7 @@ def _edge_inputs():
 
 
 def test_model_catches_planted_faults(model_bins):
-    _, selftest, _ = model_bins
+    selftest = model_bins[1]
     r = subprocess.run([selftest, "sum"], capture_output=True, text=True, env=ENV)
     assert r.returncode == 0 and r.stdout.split() == ["2016.0"] * 4, r.stdout + r.stderr
     for case, needles in (("barrier", ["left the kernel"]), ("divergent", ["lanes at a wave op"]),
