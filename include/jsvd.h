@@ -127,14 +127,16 @@ typedef struct jsvd_mixed_config
     int32_t orthogonalize_steps; /* Newton-Schulz steps applied to the FP32 V in FP64 (default 2) */
     int32_t block_width;         /* columns per block: 16, 32 or 64; 0 (default) = jsvd_autotune's choice for the
                                     shape when it measured one, else 32; halved while n <= width */
-    int32_t qr_precondition;     /* mixed path: 1 (default) = factor A = Q R in FP64 first and sweep the n x n
+    int32_t qr_precondition;     /* mixed path: 1 = factor A = Q R in FP64 first and sweep the n x n
                                     R^T (sigma_j = |x_j|, V from the normalised columns X, U = Q V_M); 2 = also
                                     factor R^T = Q2 R2 and sweep R2^T (an LQ step of R: the same singular values,
                                     columns closer to orthogonal; U = Q Y, V = Q2 V_N): 3-4 fewer FP32 sweeps on
                                     kappa = 1e8 inputs with random singular vectors (4096 x 1024: 19.6-20.1 ms
                                     against 21.1-21.3 ms), none saved on well-conditioned ones, and the second
-                                    QR costs about 2.4 ms at n = 1024; 0 = sweep A.  A factor that breaks down
-                                    falls back to the next simpler form. */
+                                    QR costs about 2.4 ms at n = 1024; 3 (default) = 1, plus the second QR when
+                                    n >= 512 and max|r_ii| / min|r_ii| of the first R is >= 100 (results equal
+                                    bit for bit those of the form chosen); 0 = sweep A.  A factor that breaks
+                                    down falls back to the next simpler form. */
     int32_t inner_order;         /* block rounds at width 32: JSVD_INNER_CROSS (default) = only the pairs across
                                     the two blocks (32 inner rounds), plus one step per sweep that sweeps every
                                     block by itself; JSVD_INNER_FULL = one cyclic sweep over all 64 columns of the

@@ -578,9 +578,9 @@ namespace
         {
             jsvd::throw_invalid("block_width must be 0, 16, 32 or 64.");
         }
-        if (mx.qr_precondition < 0 || mx.qr_precondition > 2)
+        if (mx.qr_precondition < 0 || mx.qr_precondition > 3)
         {
-            jsvd::throw_invalid("qr_precondition must be 0, 1 or 2.");
+            jsvd::throw_invalid("qr_precondition must be 0, 1, 2 or 3.");
         }
         if (mx.inner_order != JSVD_INNER_CROSS && mx.inner_order != JSVD_INNER_FULL)
         {

@@ -145,7 +145,7 @@ void jsvd_mixed_config_default(jsvd_mixed_config *mixed)
     mixed->fp32_max_sweeps = 40;
     mixed->fp32_tolerance = 0.0;
     mixed->orthogonalize_steps = 2;
-    mixed->qr_precondition = 1;
+    mixed->qr_precondition = 3;
     mixed->fp32_stop_fraction = 0.05;
 }
 

@@ -131,6 +131,36 @@ namespace jsvd
             mo[e] = (i < n && j < n && j <= i) ? r[i * ldr + j] * d[i] : 0.0;
         }
 
+        // *spread = max_i |r_ii| / min_i |r_ii| over the n diagonal entries of r (leading dimension ldr): a cheap
+        // estimate of the condition number of the column-scaled matrix r factors (+inf where an entry is zero).
+        // One workgroup of 256 threads.
+        __global__ __launch_bounds__(256) void diag_spread_kernel(const double *__restrict__ r, int64_t ldr, int64_t n,
+                                                                  double *__restrict__ spread)
+        {
+            __shared__ double hi_w[4], lo_w[4];
+            double hi = 0.0, lo = HUGE_VAL;
+            for (int64_t i = threadIdx.x; i < n; i += 256)
+            {
+                const double d = fabs(r[i * ldr + i]);
+                hi = fmax(hi, d);
+                lo = fmin(lo, d);
+            }
+            hi = dev::wave_max(hi);
+            lo = -dev::wave_max(-lo);
+            if ((threadIdx.x % 64) == 0)
+            {
+                hi_w[threadIdx.x / 64] = hi;
+                lo_w[threadIdx.x / 64] = lo;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                hi = fmax(fmax(hi_w[0], hi_w[1]), fmax(hi_w[2], hi_w[3]));
+                lo = fmin(fmin(lo_w[0], lo_w[1]), fmin(lo_w[2], lo_w[3]));
+                *spread = (lo > 0.0) ? hi / lo : HUGE_VAL;
+            }
+        }
+
         // Row-major u (m x n): zero column j where sigma_j <= eps (the reference's U cutoff).
         __global__ void mask_columns_kernel(double *__restrict__ u, int64_t m, int64_t n,
                                             const double *__restrict__ sigma, double eps)
@@ -197,7 +227,9 @@ namespace jsvd
         {
             JSVD_HIP_CHECK(hipMalloc(&a64b_, sizeof(double) * a_elems));
             JSVD_HIP_CHECK(hipMalloc(&w64_, sizeof(double) * v_elems));
-            if (qr_first() && mixed_.qr_precondition == 2)
+            JSVD_HIP_CHECK(hipMalloc(&spread_, sizeof(double)));
+            if (qr_first() && (mixed_.qr_precondition == 2 ||
+                               (mixed_.qr_precondition == 3 && ga_.n >= kSecondQrMinColumns)))
             {
                 // the second QR's Q2 and N = R2^T, in the sweep geometry (ldv x n_pad covers it: ldv >= lda of an
                 // n x n sweep, both multiples of the slab height)
@@ -226,7 +258,7 @@ namespace jsvd
 
     BlockSolver::~BlockSolver()
     {
-        for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(q2_), static_cast<void *>(n2_), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
+        for (void *p : {static_cast<void *>(dinv_), static_cast<void *>(dws_.work), static_cast<void *>(q2_), static_cast<void *>(n2_), static_cast<void *>(spread_), static_cast<void *>(a64_), static_cast<void *>(a64b_), static_cast<void *>(v64_),
                         static_cast<void *>(w64_), static_cast<void *>(gram64_), static_cast<void *>(a32_),
                         static_cast<void *>(v32_), static_cast<void *>(partial_), static_cast<void *>(q_),
                         static_cast<void *>(flags_), static_cast<void *>(rot_), static_cast<void *>(cont_), static_cast<void *>(off_),
@@ -391,7 +423,7 @@ namespace jsvd
                         g.lda, 0, dws_, stream_);
     }
 
-    bool BlockSolver::factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out)
+    bool BlockSolver::factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out, double *spread)
     {
         const int64_t ldr = cholesky_padded_size(g_.n_pad);
         hipLaunchKernelGGL(column_scale_kernel, dim3(static_cast<unsigned>(g_.n_pad)), dim3(256), 0, stream_, x,
@@ -430,6 +462,13 @@ namespace jsvd
                                               stream_));
             }
         }
+        if (spread != nullptr)
+        {
+            hipLaunchKernelGGL(diag_spread_kernel, dim3(1), dim3(256), 0, stream_, static_cast<const double *>(racc_),
+                               static_cast<int64_t>(ldr), n, spread_);
+            JSVD_HIP_CHECK(hipGetLastError());
+            JSVD_HIP_CHECK(hipMemcpyAsync(spread, spread_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        }
         int status[3] = {0, 0, 0};
         JSVD_HIP_CHECK(hipMemcpyAsync(status, chol_info_, sizeof(status), hipMemcpyDeviceToHost, stream_));
         JSVD_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -448,20 +487,27 @@ namespace jsvd
                                      jsvd_solve_info &info)
     {
         launch_transpose(d_a, ga_.n, a64_, ga_.lda, ga_.m, ga_.n, stream_);
-        // A = Q R: Q in a64_, M = R^T in a64b_
-        if (!factor_qr(a64_, ga_.m, ga_.n, ga_.lda, a64b_))
+        // A = Q R: Q in a64_, M = R^T in a64b_; the spread of R's diagonal estimates the conditioning
+        double spread = 0.0;
+        if (!factor_qr(a64_, ga_.m, ga_.n, ga_.lda, a64b_, &spread))
         {
             return false; // the caller solves A directly
         }
         // M = Q2 R2 on a copy of M: Q2 in q2_, N = R2^T in n2_; the sweeps run on N, or on M when a factor of the
-        // second QR breaks down
+        // second QR breaks down.  Automatic mode: the second QR where its three Cholesky factorisations cost less
+        // than the sweeps it saves -- n >= 512 (a sweep has n / kb - 1 outer rounds) and an ill-conditioned R
+        // (diagonal spread >= 100; measured at 4096 x 1024: 0.3-0.9 ms faster from kappa 1e4 to 1e8, 0.9-1.2 ms
+        // slower on well-conditioned Gaussian inputs; at n = 256 slower at every kappa).
         const double *sweep_in = a64b_;
         bool second = false;
-        if (mixed_.qr_precondition == 2 && q2_ != nullptr)
+        const bool want_second = mixed_.qr_precondition == 2 ||
+                                 (mixed_.qr_precondition == 3 && ga_.n >= kSecondQrMinColumns &&
+                                  spread >= kSecondQrMinSpread);
+        if (want_second && q2_ != nullptr)
         {
             JSVD_HIP_CHECK(hipMemcpyAsync(q2_, a64b_, sizeof(double) * g_.lda * g_.n_pad, hipMemcpyDeviceToDevice,
                                           stream_));
-            second = factor_qr(q2_, g_.n, g_.n, g_.lda, n2_);
+            second = factor_qr(q2_, g_.n, g_.n, g_.lda, n2_, nullptr);
             sweep_in = second ? n2_ : a64b_;
         }
         // FP32 phase, then X = N V (or M V), FP64 phase

@@ -15,12 +15,18 @@ namespace jsvd
     // tolerance, the FP32 V re-orthogonalised in FP64 (Newton-Schulz), M V formed in FP64, FP64 block sweeps to
     // the caller's epsilon; then sigma_j = |x_j|, V_A = X / sigma (X = the converged M V) and U_A = Q V_M.  Sweeping
     // R^T instead of A diagonalises R R^T, one QR-algorithm step further along than A^T A, which cuts the sweep
-    // count of both phases; the sweeps also run on n instead of m rows.  qr_precondition = 2 adds a second QR,
+    // count of both phases; the sweeps also run on n instead of m rows.  qr_precondition = 2 adds a second QR
+    // (3, the default, adds it where it pays: n >= 512 and an ill-conditioned R),
     // M = Q2 R2 (the same column-scaled shifted Cholesky QR), and sweeps N = R2^T instead (one step further again,
     // the LQ step of preconditioned Jacobi); then Y = X / sigma, U_A = Q Y and V_A = Q2 V.  When a factor of the
     // second QR breaks down the solve goes on with one QR; when one of the first breaks down or a sigma comes out
     // exactly zero (V_A cannot be normalised) the solve is redone on A directly (FP32 sweeps, Newton-Schulz, A V,
     // FP64 sweeps) with the reference's output conventions.  qr_precondition = 0 selects that direct form outright.
+    // Automatic second QR (qr_precondition 3): only for n >= this many columns and an R whose diagonal spread
+    // max |r_ii| / min |r_ii| is at least kSecondQrMinSpread.
+    constexpr int64_t kSecondQrMinColumns = 512;
+    constexpr double kSecondQrMinSpread = 100.0;
+
     class BlockSolver
     {
     public:
@@ -53,7 +59,8 @@ namespace jsvd
         void fp32_phase(const BlockGeometry &g, const double *x64, double *y64, double tol, jsvd_solve_info &info);
         // Column-scaled shifted Cholesky QR of x (m x n, leading dimension lda): x <- Q, (R D)^T into m_out (sweep
         // geometry).  False when a Cholesky factor breaks down.
-        bool factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out);
+        // With spread, also max |r_ii| / min |r_ii| of R (read back with the factorisation's status).
+        bool factor_qr(double *x, int64_t m, int64_t n, int64_t lda, double *m_out, double *spread);
         void solve_direct(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
         bool solve_qr_first(const double *d_a, double *d_u, double *d_sigma, double *d_v, jsvd_solve_info &info);
 
@@ -68,6 +75,7 @@ namespace jsvd
         double *a64b_ = nullptr;
         double *q2_ = nullptr;       // second QR of the mixed path: Q2 (n x n, sweep geometry), or null
         double *n2_ = nullptr;       // second QR: N = (R2 D2)^T, the matrix the sweeps run on
+        double *spread_ = nullptr;   // [1] diagonal spread of the first QR's R
         double *v64_ = nullptr;
         double *w64_ = nullptr;
         double *gram64_ = nullptr;

@@ -99,7 +99,8 @@ def mixed_config(path: int = PATH_MIXED, fp32_max_sweeps: Optional[int] = None,
     if inner_order is not None:
         mx.inner_order = inner_order
     if qr_precondition is not None:
-        # 0 = sweep A, 1 = one QR (sweep R^T), 2 = two QRs (sweep R2^T); a bool selects 0 or 1
+        # 0 = sweep A, 1 = one QR (sweep R^T), 2 = two QRs (sweep R2^T), 3 = two QRs when n >= 512 and R's
+        # diagonal spread is >= 100, else one; a bool selects 0 or 1
         mx.qr_precondition = int(qr_precondition)
     if block_width is not None:
         mx.block_width = block_width

@@ -403,4 +403,30 @@ def test_mixed_second_qr(torch_cuda, m, n, kind):
         assert np.max(np.abs(s2 - exact)) / exact[0] < 1e-12
         assert i2.sweeps_fp32 <= i1.sweeps_fp32, (i1.as_dict(), i2.as_dict())
     with pytest.raises(ValueError, match="qr_precondition"):
-        jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=3))
+        jsvd.Plan(m, n, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=4))
+
+
+@pytest.mark.parametrize("m,n,kind,expected", [(2048, 512, "geometric", 2), (1024, 256, "geometric", 1),
+                                               (1200, 600, "gaussian", 1)])
+def test_mixed_second_qr_automatic(torch_cuda, m, n, kind, expected):
+    """qr_precondition 3 (the default) takes the second QR only for n >= 512 and an ill-conditioned R (diagonal
+    spread >= 100): its results equal, bit for bit, those of the form it chose -- two QRs on a kappa = 1e8 matrix of
+    512 columns, one QR on the same construction at 256 columns and on a well-conditioned Gaussian of 600."""
+    rng = np.random.default_rng(17)
+    if kind == "geometric":
+        q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+        q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        a = (q1 * 10.0 ** (-8.0 * np.arange(n) / (n - 1))) @ q2.T
+    else:
+        a = rng.standard_normal((m, n))
+    cfg = jsvd.JacobiSvdConfig(epsilon=1e-14, max_sweeps=60)
+    assert jsvd.mixed_config(jsvd.PATH_MIXED).qr_precondition == 3
+    print(f"auto second qr: {m}x{n} {kind}", flush=True)
+    got = {}
+    for qr in (3, expected):
+        r, info = jsvd.solve(a, cfg, path=jsvd.PATH_MIXED, mixed=jsvd.mixed_config(jsvd.PATH_MIXED, qr_precondition=qr))
+        got[qr] = (r, info.as_dict())
+    (ra, ia), (re, ie) = got[3], got[expected]
+    assert ia == ie
+    for x, y in ((ra.u, re.u), (ra.sigma, re.sigma), (ra.v, re.v)):
+        assert np.array_equal(np.asarray(x).view(np.uint64), np.asarray(y).view(np.uint64))
